@@ -1,0 +1,60 @@
+"""Drop-in replacements of DFTBaby's iterative TD-DFTB solvers (DFTB/Solver.py).
+
+Same function names, positional/keyword arguments and return values as the
+reference (numpy arrays in, ``(Omega[k], C[k,o,v], XmY[k,o,v], XpY[k,o,v])``
+out), computed on the GPU by ``dftbaby_amd.engine``.  The extra keyword
+``precision`` ("fp64" default, "fp32", "bf16x3") selects the arithmetic of the
+(A+-B) products; the Davidson residual, orthonormalisation and subspace
+eigensolve are FP64 in every mode.
+
+Reference: HermitianDavidson Solver.py:404-578, nonHermitianDavidson
+Solver.py:637-917, exceptions Solver.py:27-33.
+"""
+import numpy as np
+
+from .engine import (CasidaOperatorGPU, ExcitedStatesError, ExcitedStatesNotConverged,  # noqa: F401
+                     hermitian_davidson_gpu, nonhermitian_davidson_gpu)
+
+_PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, 0: 0, 1: 1, 2: 2}
+
+
+def precision_code(precision):
+    if precision not in _PRECISIONS:
+        raise ValueError("precision must be 'fp64', 'fp32' or 'bf16x3', not %r" % (precision,))
+    return _PRECISIONS[precision]
+
+
+def _out(op, tensors):
+    return tuple(op.from_padded(t).cpu().numpy().copy() for t in tensors)
+
+
+def HermitianDavidson(g0, q_ov, omega, omega_shift, nocc, nvirt, XmYguess, XpYguess, Oia, selector=None,
+                      multiplicity="S", nstates=4, ifact=1, conv=1.0e-14, maxiter=10, L2_thresh=0.5, verbose=1,
+                      precision="fp64"):
+    """(A-B)^{1/2}(A+B)(A-B)^{1/2} T = Omega^2 T for the lowest nstates roots (lc = 0).
+
+    As in the reference, ``multiplicity`` is not used (singlets are computed).
+    """
+    if selector is not None:
+        raise NotImplementedError("symmetry selectors are not supported by the GPU solver")
+    op = CasidaOperatorGPU.from_charges(None, None, q_ov, omega, g0, 0.0 * np.asarray(g0), lc=0)
+    w, C, XmY, XpY, _ = hermitian_davidson_gpu(op, nstates, omega_shift=omega_shift, ifact=ifact, conv=conv,
+                                               maxiter=maxiter, XpYguess=XpYguess, Oia=Oia, L2_thresh=L2_thresh,
+                                               precision=precision_code(precision), verbose=verbose)
+    C, XmY, XpY = _out(op, (C, XmY, XpY))
+    return w.cpu().numpy(), C, XmY, XpY
+
+
+def nonHermitianDavidson(g0, g0_lr, q_oo, q_vv, q_ov, omega, nocc, nvirt, XmYguess, XpYguess, w_guess,
+                         selector=None, multiplicity="S", nstates=4, ifact=1, conv=1.0e-14, maxiter=10, verbose=1,
+                         lc=1, precision="fp64"):
+    """Stratmann-Scuseria-Frisch solver of the non-Hermitian Casida problem (lc = 1)."""
+    assert multiplicity == "S"
+    if selector is not None:
+        raise NotImplementedError("symmetry selectors are not supported by the GPU solver")
+    op = CasidaOperatorGPU.from_charges(q_oo, q_vv, q_ov, omega, g0, g0_lr, lc=lc)
+    w, C, XmY, XpY, _ = nonhermitian_davidson_gpu(op, nstates, ifact=ifact, conv=conv, maxiter=maxiter,
+                                                  XmYguess=XmYguess, XpYguess=XpYguess, w_guess=w_guess,
+                                                  precision=precision_code(precision), verbose=verbose)
+    C, XmY, XpY = _out(op, (C, XmY, XpY))
+    return w.cpu().numpy(), C, XmY, XpY

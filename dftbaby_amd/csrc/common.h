@@ -1,0 +1,39 @@
+// Shared definitions for the gfx950 LR-TD-DFTB kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+
+// Status codes returned by every C-ABI entry point.
+enum {
+  DFTB_OK = 0,
+  DFTB_ERR_ARG = 1,        // invalid argument (shape, pointer, mode)
+  DFTB_ERR_LAUNCH = 2,     // HIP launch / runtime error
+  DFTB_ERR_NOT_SPD = 3,    // subspace matrix (A-B projection) not positive definite
+  DFTB_ERR_WORKSPACE = 4,  // workspace too small
+};
+
+#define DFTB_CHECK_LAUNCH()                                  \
+  do {                                                       \
+    hipError_t e_ = hipGetLastError();                       \
+    if (e_ != hipSuccess) {                                  \
+      fprintf(stderr, "dftbaby_amd: %s\n", hipGetErrorString(e_)); \
+      return DFTB_ERR_LAUNCH;                                \
+    }                                                        \
+  } while (0)
+
+static inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+__device__ __forceinline__ unsigned short bf16_bits_rne(float x) {
+  unsigned int u = __float_as_uint(x);
+  unsigned int lsb = (u >> 16) & 1u;
+  u += 0x7fffu + lsb;
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_bits_to_float(unsigned short b) {
+  return __uint_as_float(((unsigned int)b) << 16);
+}

@@ -1,0 +1,31 @@
+#pragma once
+#include "common.h"
+#include "../../include/dftbaby_amd.h"
+
+int gemm_launch(const GemmArgs& p, hipStream_t st);
+int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st);
+int gemm_reduce_launch(const double* P, long sP, int nsplit, double* C, long n, double alpha, double beta,
+                       hipStream_t st);
+
+static inline GemmArgs gemm_args_default() {
+  GemmArgs p;
+  p.A = p.A2 = p.B = p.B2 = nullptr;
+  p.C = p.C2 = nullptr;
+  p.lda = p.ldb = p.ldc = 0;
+  p.sA = p.sB = p.sC = 0;
+  p.cdiv = 0;
+  p.cstr = 0;
+  p.cs2 = 1;
+  p.M = p.N = p.K = 0;
+  p.batch = 1;
+  p.ksplit = 1;
+  p.kchunk = 0;
+  p.sCk = 0;
+  p.alpha = 1.0;
+  p.beta = 0.0;
+  p.a_trans = p.b_trans = 0;
+  p.in_type = GEMM_IN_F64;
+  p.out_type = GEMM_OUT_F64;
+  p.tile = 0;
+  return p;
+}

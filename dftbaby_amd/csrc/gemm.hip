@@ -1,0 +1,216 @@
+// Batched, strided MFMA GEMM for gfx950 (FP64 and FP32 operands).
+//
+//   C[b] (+)= alpha * op(A[b]) * op(B[b]) + beta * C[b]
+//
+// It carries every dense contraction of the Casida matvec and of the
+// Davidson loop: the gamma-fold of the transition charges, the "expand"
+// and "contract" halves of the two long-range exchange terms, the Coulomb
+// projections, Gram matrices and subspace linear combinations.
+//
+// Tiling: 256 threads = 4 wave64s in a 2x2 arrangement; each wave owns a
+// (BM/2)x(BN/2) block of 16x16 MFMA tiles (v_mfma_f64_16x16x4f64 /
+// v_mfma_f32_16x16x4f32).  Operand tiles are staged K-major in LDS
+// (double buffered, one barrier per K step) so that the 16 lanes that feed
+// one MFMA row/column read 16 consecutive LDS words.
+//
+// Output modes: plain row-major, or "column remap" where output column n is
+// split as (n / cdiv, n % cdiv) and written to (n / cdiv) * cstr +
+// (n % cdiv) * cs2 + row * ldc.  The remap writes the exchange intermediates directly in the
+// [atom][row][col] order the following contraction consumes.  Split-K writes
+// per-slice partial results at C + slice * sCk (reduced by gemm_reduce).
+#include "common.h"
+#include "gemm.h"
+
+template <typename T> struct AccT;
+template <> struct AccT<double> { typedef d4_t type; };
+template <> struct AccT<float> { typedef f4_t type; };
+
+__device__ __forceinline__ d4_t mfma16(double a, double b, d4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4_t mfma16(float a, float b, f4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ long remap_addr(const GemmArgs& p, long row, long col) {
+  if (p.cdiv > 0) return (col / p.cdiv) * p.cstr + (col % p.cdiv) * p.cs2 + row * p.ldc;
+  return row * p.ldc + col;
+}
+
+__device__ __forceinline__ void store_out(const GemmArgs& p, long off, double val) {
+  if (p.out_type == GEMM_OUT_F64) {
+    double* C = (double*)p.C + off;
+    if (p.beta != 0.0) val += p.beta * (*C);
+    *C = val;
+  } else if (p.out_type == GEMM_OUT_F32) {
+    float* C = (float*)p.C + off;
+    if (p.beta != 0.0) val += p.beta * (double)(*C);
+    *C = (float)val;
+  } else {  // bf16 hi/lo pair
+    float x = (float)val;
+    unsigned short hi = bf16_bits_rne(x);
+    unsigned short lo = bf16_bits_rne(x - bf16_bits_to_float(hi));
+    ((unsigned short*)p.C)[off] = hi;
+    ((unsigned short*)p.C2)[off] = lo;
+  }
+}
+
+template <typename T, int BM, int BN, int BK, bool AT, bool BT>
+__global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs p) {
+  typedef typename AccT<T>::type V4;
+  __shared__ T As[2][BK][BM + 1];
+  __shared__ T Bs[2][BK][BN + 1];
+  constexpr int TM = BM / 32, TN = BN / 32;
+  constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const long n0 = (long)blockIdx.x * BN, m0 = (long)blockIdx.y * BM;
+  const int ks = blockIdx.z % p.ksplit;
+  const long b = blockIdx.z / p.ksplit;
+  const T* A = (const T*)p.A + b * p.sA;
+  const T* B = (const T*)p.B + b * p.sB;
+  const long kbeg = (long)ks * p.kchunk;
+  const long kend = min((long)p.K, kbeg + p.kchunk);
+  const long M = p.M, N = p.N;
+
+  T ra[NA], rb[NB];
+  auto gload = [&](long k0) {
+#pragma unroll
+    for (int e = 0; e < NA; ++e) {
+      int idx = tid + 256 * e;
+      int kk, r;
+      if (!AT) { kk = idx % BK; r = idx / BK; } else { r = idx % BM; kk = idx / BM; }
+      long m = m0 + r, k = k0 + kk;
+      T v = (T)0;
+      if (m < M && k < kend) v = AT ? A[k * p.lda + m] : A[m * p.lda + k];
+      ra[e] = v;
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      int idx = tid + 256 * e;
+      int kk, r;
+      if (!BT) { r = idx % BN; kk = idx / BN; } else { kk = idx % BK; r = idx / BK; }
+      long n = n0 + r, k = k0 + kk;
+      T v = (T)0;
+      if (n < N && k < kend) v = BT ? B[n * p.ldb + k] : B[k * p.ldb + n];
+      rb[e] = v;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < NA; ++e) {
+      int idx = tid + 256 * e;
+      int kk, r;
+      if (!AT) { kk = idx % BK; r = idx / BK; } else { r = idx % BM; kk = idx / BM; }
+      As[buf][kk][r] = ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+      int idx = tid + 256 * e;
+      int kk, r;
+      if (!BT) { r = idx % BN; kk = idx / BN; } else { kk = idx % BK; r = idx / BK; }
+      Bs[buf][kk][r] = rb[e];
+    }
+  };
+
+  V4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = (V4)(0);
+
+  const long nk = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
+  if (nk > 0) {
+    gload(kbeg);
+    sstore(0);
+    __syncthreads();
+    for (long kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      if (kt + 1 < nk) gload(kbeg + (kt + 1) * BK);
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        T af[TM], bf[TN];
+        const int kr = kk + (lane >> 4);
+#pragma unroll
+        for (int i = 0; i < TM; ++i) af[i] = As[cur][kr][wm * (BM / 2) + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) bf[j] = Bs[cur][kr][wn * (BN / 2) + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) acc[i][j] = mfma16(af[i], bf[j], acc[i][j]);
+      }
+      if (kt + 1 < nk) sstore(cur ^ 1);
+      __syncthreads();
+    }
+  }
+
+  const long cbase = b * p.sC + (long)ks * p.sCk;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        long row = m0 + wm * (BM / 2) + i * 16 + 4 * (lane >> 4) + r;
+        long col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
+        if (row < M && col < N) store_out(p, cbase + remap_addr(p, row, col), p.alpha * (double)acc[i][j][r]);
+      }
+}
+
+// Sum split-K partials: C[b][i] = alpha * sum_s P[s][b][i] + beta * C[b][i]  (dense, row-major M x N)
+__global__ void gemm_reduce_kernel(const double* P, long sP, int nsplit, double* C, long n, double alpha, double beta) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < nsplit; ++k) s += P[k * sP + i];
+  s *= alpha;
+  if (beta != 0.0) s += beta * C[i];
+  C[i] = s;
+}
+
+template <typename T, int BM, int BN, int BK>
+static void launch_fp(const GemmArgs& p, hipStream_t st) {
+  dim3 grid(ceil_div(p.N, BN), ceil_div(p.M, BM), p.batch * p.ksplit);
+  if (!p.a_trans && !p.b_trans) hipLaunchKernelGGL((gemm_fp_kernel<T, BM, BN, BK, false, false>), grid, dim3(256), 0, st, p);
+  else if (p.a_trans && !p.b_trans) hipLaunchKernelGGL((gemm_fp_kernel<T, BM, BN, BK, true, false>), grid, dim3(256), 0, st, p);
+  else if (!p.a_trans && p.b_trans) hipLaunchKernelGGL((gemm_fp_kernel<T, BM, BN, BK, false, true>), grid, dim3(256), 0, st, p);
+  else hipLaunchKernelGGL((gemm_fp_kernel<T, BM, BN, BK, true, true>), grid, dim3(256), 0, st, p);
+}
+
+int gemm_launch(const GemmArgs& p0, hipStream_t st) {
+  GemmArgs p = p0;
+  if (p.M <= 0 || p.N <= 0 || p.batch <= 0) return DFTB_OK;
+  if (p.K < 0 || !p.A || !p.B || !p.C) return DFTB_ERR_ARG;
+  if (p.ksplit <= 0) p.ksplit = 1;
+  if (p.kchunk <= 0) p.kchunk = p.K;
+  if (p.out_type == GEMM_OUT_BF16PAIR && !p.C2) return DFTB_ERR_ARG;
+  const bool big = (p.M >= 128 && p.N >= 128) || p.tile == 128;
+  const bool small = p.tile == 64 || !big;
+  if (p.in_type == GEMM_IN_F64) {
+    if (small) launch_fp<double, 64, 64, 16>(p, st);
+    else launch_fp<double, 128, 128, 16>(p, st);
+  } else if (p.in_type == GEMM_IN_F32) {
+    if (small) launch_fp<float, 64, 64, 32>(p, st);
+    else launch_fp<float, 128, 128, 32>(p, st);
+  } else if (p.in_type == GEMM_IN_BF16X3) {
+    return gemm_bf16x3_launch(p, st);
+  } else {
+    return DFTB_ERR_ARG;
+  }
+  DFTB_CHECK_LAUNCH();
+  return DFTB_OK;
+}
+
+int gemm_reduce_launch(const double* P, long sP, int nsplit, double* C, long n, double alpha, double beta, hipStream_t st) {
+  if (n <= 0) return DFTB_OK;
+  hipLaunchKernelGGL(gemm_reduce_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, P, sP, nsplit, C, n, alpha, beta);
+  DFTB_CHECK_LAUNCH();
+  return DFTB_OK;
+}
+
+extern "C" int dftb_gemm(const GemmArgs* p, void* stream) {
+  if (!p) return DFTB_ERR_ARG;
+  return gemm_launch(*p, (hipStream_t)stream);
+}

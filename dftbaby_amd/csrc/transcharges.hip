@@ -1,0 +1,202 @@
+// (a) Mulliken transition charges, gamma fold and operand conversions.
+//
+// q^{pq}_A = 1/2 sum_{mu in A} [X_{mu p} Y_{mu q} + X_{mu q} Y_{mu p}],  X = C, Y = S C
+// (DFTB/extensions/tddftb.f90:50-110).  S C is one MFMA GEMM; the charge kernel
+// is bound by its HBM writes (Nat * P * Q outputs, each 2*nmu FMAs).  One
+// workgroup owns one atom A and a block of PB p-orbitals: the atom's rows of
+// X and Y for those p are staged in LDS once, then every thread streams over
+// q reading X_{mu q}, Y_{mu q} (coalesced, reused PB times) and writes the PB
+// outputs q^{p q}_A for its q (coalesced along q).
+#include <vector>
+#include "common.h"
+#include "gemm.h"
+
+#define TC_PB 16
+#define TC_MAXMU 16
+
+template <int NMU>
+__global__ __launch_bounds__(256) void tc_kernel(const double* __restrict__ Xp, const double* __restrict__ Yp, int P,
+                                                 const double* __restrict__ Xq, const double* __restrict__ Yq, int Q,
+                                                 const int* __restrict__ atom_off, int Ppad, int Qpad,
+                                                 double* __restrict__ out, long sP, long sA) {
+  const int A = blockIdx.x;
+  const int p0 = blockIdx.y * TC_PB;
+  const int mu0 = atom_off[A];
+  const int nmu = atom_off[A + 1] - mu0;
+  __shared__ double sx[NMU][TC_PB];
+  __shared__ double sy[NMU][TC_PB];
+  for (int t = threadIdx.x; t < NMU * TC_PB; t += blockDim.x) {
+    int mu = t / TC_PB, pp = t % TC_PB, p = p0 + pp;
+    bool ok = (mu < nmu) && (p < P);
+    sx[mu][pp] = ok ? Xp[(long)(mu0 + mu) * P + p] : 0.0;
+    sy[mu][pp] = ok ? Yp[(long)(mu0 + mu) * P + p] : 0.0;
+  }
+  __syncthreads();
+  const int pend = min(Ppad, p0 + TC_PB);
+  for (int q = threadIdx.x; q < Qpad; q += blockDim.x) {
+    double xq[NMU], yq[NMU];
+    const bool inq = q < Q;
+#pragma unroll
+    for (int mu = 0; mu < NMU; ++mu) {
+      bool ok = inq && (mu < nmu);
+      xq[mu] = ok ? Xq[(long)(mu0 + mu) * Q + q] : 0.0;
+      yq[mu] = ok ? Yq[(long)(mu0 + mu) * Q + q] : 0.0;
+    }
+    for (int p = p0; p < pend; ++p) {
+      const int pp = p - p0;
+      double s = 0.0;
+#pragma unroll
+      for (int mu = 0; mu < NMU; ++mu) s += sx[mu][pp] * yq[mu] + xq[mu] * sy[mu][pp];
+      out[(long)p * sP + (long)A * sA + q] = (p < P && inq) ? 0.5 * s : 0.0;
+    }
+  }
+}
+
+struct TcPlan {
+  int nat, norb, nocc, nvirt, maxmu;
+  const double *Co, *Cv;
+  double *Yo, *Yv;
+  int* off;
+};
+
+static int tc_block(const TcPlan& t, bool p_occ, bool q_occ, int Ppad, int Qpad, double* out, long sP, long sA,
+                    hipStream_t st) {
+  if (!out) return DFTB_OK;
+  const double* Xp = p_occ ? t.Co : t.Cv;
+  const double* Yp = p_occ ? t.Yo : t.Yv;
+  const double* Xq = q_occ ? t.Co : t.Cv;
+  const double* Yq = q_occ ? t.Yo : t.Yv;
+  int P = p_occ ? t.nocc : t.nvirt;
+  int Q = q_occ ? t.nocc : t.nvirt;
+  if (P == 0 || Ppad == 0 || Qpad == 0) return DFTB_OK;
+  dim3 grid(t.nat, ceil_div(Ppad, TC_PB));
+  if (t.maxmu <= 4)
+    hipLaunchKernelGGL(tc_kernel<4>, grid, dim3(256), 0, st, Xp, Yp, P, Xq, Yq, Q, t.off, Ppad, Qpad, out, sP, sA);
+  else if (t.maxmu <= 9)
+    hipLaunchKernelGGL(tc_kernel<9>, grid, dim3(256), 0, st, Xp, Yp, P, Xq, Yq, Q, t.off, Ppad, Qpad, out, sP, sA);
+  else
+    hipLaunchKernelGGL(tc_kernel<16>, grid, dim3(256), 0, st, Xp, Yp, P, Xq, Yq, Q, t.off, Ppad, Qpad, out, sP, sA);
+  DFTB_CHECK_LAUNCH();
+  return DFTB_OK;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+extern "C" size_t dftb_trans_charges_workspace(int norb, int nocc, int nvirt, int nat) {
+  return align256((size_t)norb * (nocc + nvirt) * sizeof(double)) + align256((size_t)(nat + 1) * sizeof(int));
+}
+
+// Prepare S C products and atom offsets (orbs_per_atom is a host array, as the
+// f2py argument it replaces).
+static int tc_prepare(TcPlan& t, const double* Co, const double* Cv, const double* S, const int* orbs_per_atom,
+                      int nat, int norb, int nocc, int nvirt, void* ws, size_t ws_bytes, hipStream_t st) {
+  if (nat <= 0 || norb <= 0 || nocc < 0 || nvirt < 0 || !S || !orbs_per_atom || !ws) return DFTB_ERR_ARG;
+  if (ws_bytes < dftb_trans_charges_workspace(norb, nocc, nvirt, nat)) return DFTB_ERR_WORKSPACE;
+  std::vector<int> off(nat + 1, 0);
+  int maxmu = 0;
+  for (int A = 0; A < nat; ++A) {
+    if (orbs_per_atom[A] < 0 || orbs_per_atom[A] > TC_MAXMU) return DFTB_ERR_ARG;
+    off[A + 1] = off[A] + orbs_per_atom[A];
+    if (orbs_per_atom[A] > maxmu) maxmu = orbs_per_atom[A];
+  }
+  if (off[nat] != norb) return DFTB_ERR_ARG;
+  char* w = (char*)ws;
+  t.nat = nat; t.norb = norb; t.nocc = nocc; t.nvirt = nvirt; t.maxmu = maxmu;
+  t.Co = Co; t.Cv = Cv;
+  t.Yo = (double*)w;
+  t.Yv = t.Yo + (size_t)norb * nocc;
+  t.off = (int*)(w + align256((size_t)norb * (nocc + nvirt) * sizeof(double)));
+  if (hipMemcpyAsync(t.off, off.data(), (nat + 1) * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+    return DFTB_ERR_LAUNCH;
+  // Y = S C  (S symmetric, row-major [norb][norb]; C row-major [norb][n])
+  for (int pass = 0; pass < 2; ++pass) {
+    int n = pass == 0 ? nocc : nvirt;
+    if (n == 0) continue;
+    GemmArgs g = gemm_args_default();
+    g.A = S; g.lda = norb;
+    g.B = pass == 0 ? Co : Cv; g.ldb = n;
+    g.C = pass == 0 ? t.Yo : t.Yv; g.ldc = n;
+    g.M = norb; g.N = n; g.K = norb;
+    int rc = gemm_launch(g, st);
+    if (rc) return rc;
+  }
+  return DFTB_OK;
+}
+
+extern "C" int dftb_trans_charges(const double* orbs_occ, const double* orbs_virt, const double* S,
+                                  const int* orbs_per_atom, int nat, int norb, int nocc, int nvirt, double* q_oo,
+                                  double* q_ov, double* q_vv, void* workspace, size_t ws_bytes, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  TcPlan t;
+  int rc = tc_prepare(t, orbs_occ, orbs_virt, S, orbs_per_atom, nat, norb, nocc, nvirt, workspace, ws_bytes, st);
+  if (rc) return rc;
+  long o = nocc, v = nvirt;
+  if ((rc = tc_block(t, true, true, nocc, nocc, q_oo, o, o * o, st))) return rc;
+  if ((rc = tc_block(t, true, false, nocc, nvirt, q_ov, v, o * v, st))) return rc;
+  if ((rc = tc_block(t, false, false, nvirt, nvirt, q_vv, v, v * v, st))) return rc;
+  return DFTB_OK;
+}
+
+static inline int pad8(int n) { return (n + 7) & ~7; }
+
+extern "C" int dftb_trans_charges_engine(const double* orbs_occ, const double* orbs_virt, const double* S,
+                                         const int* orbs_per_atom, int nat, int norb, int nocc, int nvirt,
+                                         double* q_oo_e, double* q_ov_e, double* q_vv_e, double* q_c,
+                                         void* workspace, size_t ws_bytes, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  TcPlan t;
+  int rc = tc_prepare(t, orbs_occ, orbs_virt, S, orbs_per_atom, nat, norb, nocc, nvirt, workspace, ws_bytes, st);
+  if (rc) return rc;
+  long opad = pad8(nocc), vpad = pad8(nvirt), na = nat;
+  if ((rc = tc_block(t, true, true, nocc, (int)opad, q_oo_e, na * opad, opad, st))) return rc;
+  if ((rc = tc_block(t, true, false, nocc, (int)vpad, q_ov_e, na * vpad, vpad, st))) return rc;
+  if ((rc = tc_block(t, false, false, (int)vpad, (int)vpad, q_vv_e, na * vpad, vpad, st))) return rc;
+  if ((rc = tc_block(t, true, false, (int)opad, (int)vpad, q_c, vpad, opad * vpad, st))) return rc;
+  return DFTB_OK;
+}
+
+extern "C" int dftb_fold_gamma(const double* gamma, int nat, const double* q, long np, long qpad, double* qf,
+                               void* stream) {
+  if (!gamma || !q || !qf || nat <= 0) return DFTB_ERR_ARG;
+  GemmArgs g = gemm_args_default();
+  g.A = gamma; g.lda = nat; g.sA = 0;
+  g.B = q; g.ldb = qpad; g.sB = (long)nat * qpad;
+  g.C = qf; g.ldc = qpad; g.sC = (long)nat * qpad;
+  g.M = nat; g.N = (int)qpad; g.K = nat; g.batch = (int)np;
+  return gemm_launch(g, (hipStream_t)stream);
+}
+
+__global__ void cvt_f32_kernel(const double* x, long n, float* y) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+
+__global__ void cvt_bf16pair_kernel(const double* x, long n, unsigned short* hi, unsigned short* lo) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    float f = (float)x[i];
+    unsigned short h = bf16_bits_rne(f);
+    hi[i] = h;
+    lo[i] = bf16_bits_rne(f - bf16_bits_to_float(h));
+  }
+}
+
+static dim3 stream_grid(long n) {
+  long b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+
+extern "C" int dftb_convert_f32(const double* x, long n, float* y, void* stream) {
+  if (n <= 0) return DFTB_OK;
+  hipLaunchKernelGGL(cvt_f32_kernel, stream_grid(n), dim3(256), 0, (hipStream_t)stream, x, n, y);
+  DFTB_CHECK_LAUNCH();
+  return DFTB_OK;
+}
+
+extern "C" int dftb_convert_bf16pair(const double* x, long n, unsigned short* hi, unsigned short* lo, void* stream) {
+  if (n <= 0) return DFTB_OK;
+  hipLaunchKernelGGL(cvt_bf16pair_kernel, stream_grid(n), dim3(256), 0, (hipStream_t)stream, x, n, hi, lo);
+  DFTB_CHECK_LAUNCH();
+  return DFTB_OK;
+}

@@ -1,0 +1,363 @@
+"""Host-side SCC ground-state producer (the input side of the excited-state path).
+
+The excited-state engine consumes a converged closed-shell ground state through
+the getters that ``DFTB/DFTB2.py:2405-2454`` exposes to ``LR_TDDFTB``
+(``getKSCoefficients``, ``getKSEnergies``, ``getOccupation``,
+``getOverlapMatrix``, ``getGamma``, ``getGamma_lr``, ``getGaussianOverlap``,
+``getDipoleMatrix``, ``getGeometry``, ``getValorbs``, ``orbsPerAtom``,
+``Nelec_val``).  This module produces such a state for H/C molecules with the
+reference's "homegrown" parameters (converted by ``make_slako_data``):
+
+* H0 and S from the Slater-Koster tables with the two-centre s/p rules
+  (``DFTB/extensions/slako.f90:77-231``).  Table values are interpolated with
+  a not-a-knot cubic spline (the interpolant of ``scipy.interpolate.splrep``
+  with s=0) and are zero beyond the last grid point, as in the reference's
+  Python path (``DFTB/SlaterKoster/SKIntegrals.py:823-824``).  The reference's
+  default Fortran spline path (``splines.f90:75``) differs from this by ~1e-5
+  in S; the excited-state engine is checked on identical inputs, so this only
+  affects end-to-end comparisons with DFTBaby itself.
+* gamma and gamma_lr for Gaussian charge fluctuations
+  (``DFTB/GammaApproximation.py:8-82,122-151,210-243,388-405``), sigma = 1/(sqrt(pi) U).
+* SCC iterations with Mulliken charges and, for ``long_range_correction=1``,
+  the long-range exchange operator on the difference density
+  (``DFTB/DFTB2.py:1240-1266``), accelerated by Pulay DIIS on the Fock matrix.
+* T=0 occupations (``DFTB/DFTB2.py:2594-2610``).
+* The Gaussian overlap Omega_AB for the Lambda2 diagnostic
+  (``DFTB/DFTB2.py:1287-1317``).
+* The AO dipole matrix in the Mulliken point-charge approximation
+  D^c_{mu nu} = S_{mu nu} (R^c_A + R^c_B)/2.  The reference's default uses
+  Slater-Koster dipole tables; this producer does not, so oscillator strengths
+  computed from it equal the Mulliken-approximation ones.
+
+The repulsive potential is not included: excitation energies do not depend on
+it (``DFTB/DFTB2.py:100-102``).
+"""
+import math
+import os
+
+import numpy as np
+import torch
+from scipy.interpolate import CubicSpline
+from scipy.special import erf
+
+_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "slako_hc.npz")
+
+
+class SCFNotConverged(Exception):
+    pass
+
+
+class _SlakoTables:
+    def __init__(self):
+        d = np.load(_DATA)
+        self.U = {1: float(d["h_U"]), 6: float(d["c_U"])}
+        # valence orbitals (n, l, m) per element, DFTB/DFTB2.py:240-245
+        self.valorbs = {1: [(0, 0, 0)], 6: [(1, 0, 0), (1, 1, -1), (1, 1, 0), (1, 1, 1)]}
+        self.orbital_energies = {1: [float(d["h_energy_1s"])],
+                                 6: [float(d["c_energy_2s"])] + 3 * [float(d["c_energy_2p"])]}
+        self.ne_val = {1: 1, 6: 4}
+        # reference (neutral atom) populations per orbital
+        self.occ_ref = {1: [1.0], 6: [2.0, 2.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0]}
+        self.splines = {}
+        self.dmax = {}
+        for pair, (z1, z2) in (("h_h", (1, 1)), ("h_c", (1, 6)), ("c_c", (6, 6))):
+            grid = d["%s_d" % pair]
+            self.dmax[(z1, z2)] = grid[-1]
+            for kind in ("S", "H"):
+                keys = d["%s_%s_keys" % (pair, kind)]
+                vals = d["%s_%s_vals" % (pair, kind)]
+                for k, v in zip(keys, vals):
+                    self.splines[(z1, z2, kind, tuple(int(x) for x in k))] = CubicSpline(grid, v)
+
+    def value(self, z1, z2, kind, key, r):
+        spl = self.splines.get((z1, z2, kind, key))
+        out = np.zeros_like(r)
+        if spl is None:
+            return out
+        inside = r <= self.dmax[(z1, z2)]
+        out[inside] = spl(r[inside])
+        return out
+
+
+_TABLES = None
+
+
+def _tables():
+    global _TABLES
+    if _TABLES is None:
+        _TABLES = _SlakoTables()
+    return _TABLES
+
+
+def _pair_integrals(T, kind, ZA, ZB, r):
+    """Slater-Koster integrals for atom pairs A->B (arrays over pairs).
+
+    Returns (ss, sp, ps, pp_sigma, pp_pi) where sp has s on A and p on B and
+    ps has p on A and s on B, in the orientation of the direction A->B.
+    """
+    n = len(r)
+    ss = np.zeros(n); sp = np.zeros(n); ps = np.zeros(n); pps = np.zeros(n); ppp = np.zeros(n)
+    for (za, zb) in ((1, 1), (1, 6), (6, 1), (6, 6)):
+        m = (ZA == za) & (ZB == zb)
+        if not m.any():
+            continue
+        rr = r[m]
+        if (za, zb) == (1, 1):
+            ss[m] = T.value(1, 1, kind, (0, 0, 0), rr)
+        elif (za, zb) == (1, 6):
+            ss[m] = T.value(1, 6, kind, (0, 0, 0), rr)
+            sp[m] = T.value(1, 6, kind, (0, 1, 2), rr)
+        elif (za, zb) == (6, 1):
+            ss[m] = T.value(1, 6, kind, (0, 0, 0), rr)
+            # p on C(A), s on H(B): E_{p_A s_B}(e) = E_{s_B p_A}(-e)
+            ps[m] = -T.value(1, 6, kind, (0, 1, 2), rr)
+        else:
+            ss[m] = T.value(6, 6, kind, (0, 0, 0), rr)
+            sp[m] = T.value(6, 6, kind, (0, 1, 2), rr)
+            ps[m] = T.value(6, 6, kind, (1, 0, 4), rr)
+            ppp[m] = T.value(6, 6, kind, (1, 1, 5), rr)
+            pps[m] = T.value(6, 6, kind, (1, 1, 6), rr)
+    return ss, sp, ps, pps, ppp
+
+
+class DFTB2:
+    """Minimal DFTB2 / LC-DFTB ground state for H/C molecules (see module doc)."""
+
+    def __init__(self, atomlist, long_range_correction=1, long_range_radius=3.03,
+                 distance_cutoff=30.0, verbose=0, parameter_set="homegrown",
+                 mulliken_dipoles=0, use_symmetry=0, device=None, **opts):
+        self.long_range_correction = int(long_range_correction)
+        self.long_range_radius = float(long_range_radius)
+        self.distance_cutoff = distance_cutoff
+        self.verbose = verbose
+        self.parameter_set = parameter_set
+        self.mulliken_dipoles = mulliken_dipoles
+        self.use_symmetry = use_symmetry
+        self.device = device
+        self._T = _tables()
+        self.hubbard_U = dict(self._T.U)
+        self.valorbs = self._T.valorbs
+        self.setGeometry(atomlist)
+
+    # -- geometry dependent set-up ------------------------------------------------
+    def setGeometry(self, atomlist, **opts):
+        for Z, _ in atomlist:
+            if Z not in (1, 6):
+                raise ValueError("only H and C are parametrised in this producer")
+        self.atomlist = [(int(Z), list(map(float, x))) for Z, x in atomlist]
+        self.Z = np.array([Z for Z, _ in self.atomlist])
+        self.pos = np.array([x for _, x in self.atomlist], dtype=np.float64)
+        self.orbsPerAtom = np.array([len(self.valorbs[Z]) for Z in self.Z], dtype=np.int32)
+        self.dim = int(self.orbsPerAtom.sum())
+        self.Nelec_val = int(sum(self._T.ne_val[Z] for Z in self.Z))
+        self.orb_atom = np.repeat(np.arange(len(self.Z)), self.orbsPerAtom)
+        for a in ("orbs", "orbe", "f", "S", "H0", "P"):
+            if hasattr(self, a):
+                delattr(self, a)
+
+    def _constructH0andS(self):
+        T = self._T
+        nat = len(self.Z)
+        off = np.concatenate([[0], np.cumsum(self.orbsPerAtom)[:-1]])
+        S = np.zeros((self.dim, self.dim))
+        H0 = np.zeros((self.dim, self.dim))
+        for A in range(nat):
+            o = off[A]
+            for k, e in enumerate(T.orbital_energies[self.Z[A]]):
+                S[o + k, o + k] = 1.0
+                H0[o + k, o + k] = e
+        dvec = self.pos[None, :, :] - self.pos[:, None, :]
+        r = np.linalg.norm(dvec, axis=-1)
+        iA, iB = np.where((r > 1e-8) & (r < min(self.distance_cutoff, 10.5)))
+        rr = r[iA, iB]
+        e = dvec[iA, iB] / rr[:, None]
+        # real p orbitals: m=-1 -> y, m=0 -> z, m=+1 -> x
+        ecomp = np.stack([e[:, 1], e[:, 2], e[:, 0]], axis=1)
+        ZA, ZB = self.Z[iA], self.Z[iB]
+        oA, oB = off[iA], off[iB]
+        for M, kind in ((S, "S"), (H0, "H")):
+            ss, sp, ps, pps, ppp = _pair_integrals(T, kind, ZA, ZB, rr)
+            M[oA, oB] = ss
+            pB = ZB == 6
+            pA = ZA == 6
+            for c in range(3):
+                M[oA[pB], oB[pB] + 1 + c] = ecomp[pB, c] * sp[pB]
+                M[oA[pA] + 1 + c, oB[pA]] = ecomp[pA, c] * ps[pA]
+            both = pA & pB
+            for c1 in range(3):
+                for c2 in range(3):
+                    val = ecomp[both, c1] * ecomp[both, c2] * (pps[both] - ppp[both])
+                    if c1 == c2:
+                        val = val + ppp[both]
+                    M[oA[both] + 1 + c1, oB[both] + 1 + c2] = val
+        S = 0.5 * (S + S.T)
+        H0 = 0.5 * (H0 + H0.T)
+        return S, H0
+
+    def _gamma_matrices(self):
+        U = np.array([self.hubbard_U[Z] for Z in self.Z])
+        sig = 1.0 / (np.sqrt(np.pi) * U)
+        r = np.linalg.norm(self.pos[:, None, :] - self.pos[None, :, :], axis=-1)
+        s2 = sig[:, None] ** 2 + sig[None, :] ** 2
+        off = ~np.eye(len(U), dtype=bool)
+        g = np.zeros_like(r)
+        C = 1.0 / np.sqrt(2.0 * s2)
+        g[off] = erf(C[off] * r[off]) / r[off]
+        g[~off] = 1.0 / (np.sqrt(np.pi) * sig)
+        glr = np.zeros_like(r)
+        if self.long_range_correction == 1:
+            Rlr = self.long_range_radius
+            Clr = 1.0 / np.sqrt(2.0 * (s2 + 0.5 * Rlr ** 2))
+            glr[off] = erf(Clr[off] * r[off]) / r[off]
+            glr[~off] = 1.0 / np.sqrt(np.pi * (sig ** 2 + 0.25 * Rlr ** 2))
+        return g, glr
+
+    def _construct_gaussian_overlap(self):
+        U = np.array([self.hubbard_U[Z] for Z in self.Z])
+        sig = 1.329 / math.sqrt(8.0 * math.log(2.0)) / U
+        sAB = sig[:, None] ** 2 + sig[None, :] ** 2
+        r2 = np.sum((self.pos[:, None, :] - self.pos[None, :, :]) ** 2, axis=-1)
+        return 1.0 / (2.0 * np.pi * sAB) ** 1.5 * np.exp(-0.5 * r2 / sAB)
+
+    # -- SCC --------------------------------------------------------------------
+    def getEnergy(self, scf_conv=1e-10, maxiter=300, diis_size=8, mixing=0.3, **opts):
+        dev = self.device
+        if dev is None:
+            dev = "cuda" if (torch.cuda.is_available() and self.dim > 400) else "cpu"
+        S, H0 = self._constructH0andS()
+        self.S, self.H0 = S, H0
+        self.gamma, self.gamma_lr = self._gamma_matrices()
+        self.gaussOmega = self._construct_gaussian_overlap()
+        t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev)
+        St, H0t = t(S), t(H0)
+        oa = torch.as_tensor(self.orb_atom, device=dev)
+        nat = len(self.Z)
+        G = t(self.gamma)
+        Glr_ao = t(self.gamma_lr)[oa][:, oa]
+        q0 = t([self._T.ne_val[Z] for Z in self.Z])
+        P0 = torch.diag(t(np.concatenate([self._T.occ_ref[Z] for Z in self.Z])))
+        L = torch.linalg.cholesky(St)
+        Linv = torch.linalg.solve_triangular(L, torch.eye(self.dim, dtype=torch.float64, device=dev), upper=False)
+        nocc = self.Nelec_val // 2
+        if self.Nelec_val % 2:
+            raise ValueError("closed-shell ground state needs an even electron count")
+        P = P0.clone()
+        fock_hist, err_hist = [], []
+        converged = False
+        for it in range(maxiter):
+            dq = torch.zeros(nat, dtype=torch.float64, device=dev).index_add_(0, oa, (P * St).sum(1)) - q0
+            V = G @ dq
+            F = H0t + 0.5 * St * (V[oa][:, None] + V[oa][None, :])
+            if self.long_range_correction == 1:
+                dP = P - P0
+                SdP = St @ dP
+                dPS = dP @ St
+                Hx = (Glr_ao * SdP) @ St + Glr_ao * (SdP @ St) + St @ (dP * Glr_ao) @ St + St @ (dPS * Glr_ao)
+                F = F - Hx / 8.0
+            err = F @ P @ St - St @ P @ F
+            errn = float(err.abs().max())
+            fock_hist.append(F)
+            err_hist.append(Linv @ err @ Linv.T)
+            if len(fock_hist) > diis_size:
+                fock_hist.pop(0)
+                err_hist.pop(0)
+            if it >= 2 and len(fock_hist) >= 3:
+                m = len(fock_hist)
+                B = torch.empty((m + 1, m + 1), dtype=torch.float64, device=dev)
+                for i in range(m):
+                    for j in range(i, m):
+                        B[i, j] = B[j, i] = (err_hist[i] * err_hist[j]).sum()
+                B[m, :] = -1.0
+                B[:, m] = -1.0
+                B[m, m] = 0.0
+                rhs = torch.zeros(m + 1, dtype=torch.float64, device=dev)
+                rhs[m] = -1.0
+                try:
+                    c = torch.linalg.solve(B, rhs)[:m]
+                    Fx = sum(ci * Fi for ci, Fi in zip(c, fock_hist))
+                except RuntimeError:
+                    Fx = F
+            else:
+                Fx = F
+            Fp = Linv @ Fx @ Linv.T
+            e, Cp = torch.linalg.eigh(0.5 * (Fp + Fp.T))
+            C = torch.linalg.solve_triangular(L.T, Cp, upper=True)
+            Pn = 2.0 * C[:, :nocc] @ C[:, :nocc].T
+            if it < 2:
+                Pn = mixing * Pn + (1.0 - mixing) * P if it > 0 else Pn
+            dPmax = float((Pn - P).abs().max())
+            P = Pn
+            if self.verbose > 1:
+                print("SCC it %3d  commutator %.3e  dP %.3e" % (it, errn, dPmax))
+            if errn < scf_conv ** 0.5 * 1e-2 and dPmax < scf_conv ** 0.5 * 1e-2:
+                converged = True
+                break
+        if not converged:
+            raise SCFNotConverged("SCC did not converge in %d iterations (commutator %.3e)" % (maxiter, errn))
+        self.orbe = e.cpu().numpy()
+        self.orbs = C.cpu().numpy()
+        self.P = P.cpu().numpy()
+        self.dq = dq.cpu().numpy()
+        self.scc_iterations = it + 1
+        _, self.f = fermi_occupation_T0(self.orbe, self.Nelec_val)
+        # electronic energy (band structure + second order + lc exchange), without repulsion
+        E = float((P * H0t).sum() + 0.5 * dq @ G @ dq)
+        self.en_elec = E
+        return E
+
+    # -- getters used by the excited-state path (DFTB/DFTB2.py:2405-2454) --------
+    def getKSCoefficients(self):
+        return self.orbs
+
+    def getKSEnergies(self):
+        return self.orbe
+
+    def getOccupation(self):
+        return self.f
+
+    def getOverlapMatrix(self):
+        if not hasattr(self, "S"):
+            self.S, self.H0 = self._constructH0andS()
+        return self.S
+
+    def getGamma(self):
+        return self.gamma
+
+    def getGamma_lr(self):
+        return self.gamma_lr
+
+    def getGaussianOverlap(self):
+        return self.gaussOmega
+
+    def getDipoleMatrix(self):
+        S = self.getOverlapMatrix()
+        R = self.pos[self.orb_atom]
+        return S[:, :, None] * 0.5 * (R[:, None, :] + R[None, :, :])
+
+    def getGeometry(self):
+        return self.atomlist
+
+    def getValorbs(self):
+        return self.valorbs
+
+    def getFrontierOrbitals(self):
+        HOMO = np.where(self.f > 0.0)[0][-1]
+        LUMO = np.where(self.f == 0.0)[0][0]
+        return HOMO, LUMO
+
+    def getHOMO_LUMO_gap(self):
+        H, L = self.getFrontierOrbitals()
+        return self.orbe[L] - self.orbe[H]
+
+
+def fermi_occupation_T0(orbe, Nelec_paired, Nelec_unpaired=0):
+    """T=0 aufbau occupations, DFTB/DFTB2.py:2594-2610."""
+    f = np.zeros(len(orbe))
+    for a in np.argsort(orbe, kind="stable"):
+        if Nelec_paired > 0.0:
+            f[a] = min(2.0, Nelec_paired)
+            Nelec_paired -= 2.0
+        elif Nelec_unpaired > 0.0:
+            f[a] = min(1.0, Nelec_unpaired)
+            Nelec_unpaired -= 1.0
+    return 0.0, f

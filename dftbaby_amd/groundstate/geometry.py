@@ -1,0 +1,107 @@
+"""Synthetic molecules for the excited-state workloads.
+
+Geometries are lists of ``(Z, [x, y, z])`` in bohr, the atom-list convention of
+``DFTB/XYZ.py:70`` (read_xyz converts Angstrom to bohr).
+
+* ``polyene(n)``: planar all-trans C_nH_{n+2} with C=C 1.34, C-C 1.46,
+  C-H 1.09 Angstrom and 120 degree angles.
+* ``graphene_flake(rows, cols)``: rectangular honeycomb cut of ``rows`` zigzag
+  chains with ``cols`` carbons each (C-C 1.42 Angstrom); every carbon with
+  fewer than three carbon neighbours is capped with hydrogens (C-H 1.09) along
+  the missing bond directions.
+* ``displace(atoms, sigma_angstrom, seed)``: seeded Gaussian displacement of
+  every Cartesian component.
+"""
+import numpy as np
+
+ANGSTROM = 1.0 / 0.52917721092  # bohr per Angstrom
+
+
+def _rot2(v, deg):
+    t = np.deg2rad(deg)
+    c, s = np.cos(t), np.sin(t)
+    return np.array([c * v[0] - s * v[1], s * v[0] + c * v[1]])
+
+
+def polyene(n, cc_double=1.34, cc_single=1.46, ch=1.09):
+    """All-trans C_nH_{n+2} (n even) in the xy plane, coordinates in bohr."""
+    assert n >= 2 and n % 2 == 0
+    up = np.array([np.cos(np.pi / 6), np.sin(np.pi / 6)])
+    dn = np.array([np.cos(np.pi / 6), -np.sin(np.pi / 6)])
+    cpos = [np.zeros(2)]
+    for k in range(1, n):
+        length = cc_double if (k % 2 == 1) else cc_single
+        step = up if (k % 2 == 1) else dn
+        cpos.append(cpos[-1] + length * step)
+    cpos = np.array(cpos)
+    hpos = []
+    for k in range(n):
+        nbrs = [cpos[j] for j in (k - 1, k + 1) if 0 <= j < n]
+        units = [(p - cpos[k]) / np.linalg.norm(p - cpos[k]) for p in nbrs]
+        if len(units) == 2:
+            d = -(units[0] + units[1])
+            d /= np.linalg.norm(d)
+            hpos.append(cpos[k] + ch * d)
+        else:
+            u = units[0]
+            for ang in (120.0, -120.0):
+                hpos.append(cpos[k] + ch * _rot2(u, ang))
+    atoms = [(6, [p[0], p[1], 0.0]) for p in cpos] + [(1, [p[0], p[1], 0.0]) for p in hpos]
+    return [(Z, list(np.array(x) * ANGSTROM)) for Z, x in atoms]
+
+
+def graphene_flake(rows=24, cols=25, cc=1.42, ch=1.09):
+    """Rectangular H-terminated graphene flake (rows zigzag chains x cols atoms)."""
+    a = cc
+    pos = {}
+    for r in range(rows):
+        for j in range(cols):
+            x = j * np.sqrt(3.0) / 2.0 * a
+            y = r * 1.5 * a + (0.5 * a if (j + r) % 2 == 1 else 0.0)
+            pos[(r, j)] = np.array([x, y])
+    keys = list(pos.keys())
+    index = {k: i for i, k in enumerate(keys)}
+    cxy = np.array([pos[k] for k in keys])
+    nat = len(keys)
+    d = np.linalg.norm(cxy[:, None, :] - cxy[None, :, :], axis=-1)
+    bonded = (d < 1.1 * a) & (d > 0.1)
+    hxy = []
+    for i in range(nat):
+        nb = np.where(bonded[i])[0]
+        units = [(cxy[j] - cxy[i]) / np.linalg.norm(cxy[j] - cxy[i]) for j in nb]
+        if len(units) == 2:
+            dvec = -(units[0] + units[1])
+            dvec /= np.linalg.norm(dvec)
+            hxy.append(cxy[i] + ch * dvec)
+        elif len(units) == 1:
+            for ang in (120.0, -120.0):
+                hxy.append(cxy[i] + ch * _rot2(units[0], ang))
+    if len(hxy) % 2 == 1:
+        # closed shell needs an even electron count (4 per C, 1 per H)
+        hxy = hxy[:-1]
+    del index
+    atoms = [(6, [p[0], p[1], 0.0]) for p in cxy] + [(1, [p[0], p[1], 0.0]) for p in hxy]
+    return [(Z, list(np.array(x) * ANGSTROM)) for Z, x in atoms]
+
+
+def displace(atoms, sigma_angstrom, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for Z, x in atoms:
+        out.append((Z, list(np.array(x) + rng.normal(0.0, sigma_angstrom * ANGSTROM, 3))))
+    return out
+
+
+def ethylene():
+    return polyene(2)
+
+
+def benzene(cc=1.39, ch=1.09):
+    atoms = []
+    for k in range(6):
+        t = np.deg2rad(60.0 * k)
+        atoms.append((6, [cc * np.cos(t), cc * np.sin(t), 0.0]))
+    for k in range(6):
+        t = np.deg2rad(60.0 * k)
+        atoms.append((1, [(cc + ch) * np.cos(t), (cc + ch) * np.sin(t), 0.0]))
+    return [(Z, list(np.array(x) * ANGSTROM)) for Z, x in atoms]

@@ -1,0 +1,200 @@
+/*
+ * dftbaby_amd: C ABI of the MI355X (gfx950) LR-TD-DFTB excited-state engine.
+ *
+ * Every entry point takes device pointers, plain sizes and a hipStream_t
+ * (passed as void*), returns an int status (0 = ok, see DFTB_ERR_* in
+ * csrc/common.h: 1 invalid argument, 2 launch error, 3 subspace matrix not
+ * positive definite, 4 workspace too small) and never allocates device memory:
+ * the caller owns all buffers, including the workspace.  No exception crosses
+ * the ABI.  Calls on one stream are ordered; the library keeps no state.
+ *
+ * Vector layout: an excitation vector is a dense [opad][vpad] FP64 block
+ * (row i = occupied orbital, column a = virtual orbital, idx = i*vpad + a,
+ * the reference's i-major flattening of Solver.py:242-247) whose padding
+ * (i >= nocc or a >= nvirt) is zero.  opad and vpad are nocc and nvirt rounded
+ * up to a multiple of 8.
+ */
+#ifndef DFTBABY_AMD_H
+#define DFTBABY_AMD_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- generic batched GEMM ------------------------------------------------
+ * C[b] = alpha * op(A[b]) op(B[b]) + beta * C[b] on MFMA.  Element (m,k) of A
+ * is A[b*sA + (a_trans ? k*lda + m : m*lda + k)], element (k,n) of B is
+ * B[b*sB + (b_trans ? n*ldb + k : k*ldb + n)].  With cdiv > 0 output column n
+ * is written to (n / cdiv) * cstr + (n % cdiv) * cs2 + m * ldc.  ksplit > 1 splits K
+ * into chunks of kchunk and writes slice s at C + s*sCk (beta must be 0).
+ * in_type GEMM_IN_BF16X3 takes (hi, lo) bf16 operand pairs in (A, A2) and
+ * (B, B2) and forms hi*hi + hi*lo + lo*hi with FP32 accumulation; it needs
+ * a_trans = 0, b_trans = 1 (both operands K-contiguous), K, lda, ldb and the
+ * operand offsets multiples of 8 elements.
+ * No reference counterpart: it carries the np.tensordot / MATMUL contractions
+ * of Solver.py:583-635, 759-807 and tddftb.f90:114-287. */
+enum { GEMM_IN_F64 = 0, GEMM_IN_F32 = 1, GEMM_IN_BF16X3 = 2 };
+enum { GEMM_OUT_F64 = 0, GEMM_OUT_F32 = 1, GEMM_OUT_BF16PAIR = 2 };
+
+typedef struct GemmArgs {
+  const void* A; const void* A2;
+  const void* B; const void* B2;
+  void* C; void* C2;
+  long lda, ldb, ldc;
+  long sA, sB, sC;
+  long cdiv, cstr, cs2;
+  int M, N, K;
+  int batch;
+  int ksplit, kchunk;
+  long sCk;
+  double alpha, beta;
+  int a_trans, b_trans;
+  int in_type, out_type;
+  int tile; /* 0 auto, 64 or 128 */
+} GemmArgs;
+
+int dftb_gemm(const GemmArgs* p, void* stream);
+
+/* ---- (a) Mulliken transition charges --------------------------------------
+ * Replaces f2py tddftb.trans_charges(orbs_occ, orbs_virt, S, orbsPerAtom)
+ * -> (q_oo, q_ov, q_vv)   (DFTB/extensions/tddftb.f90:12-112, called at
+ * DFTB/LR_TDDFTB.py:303-306):
+ *   q^{pq}_A = 1/2 sum_{mu in A} [C_{mu p} (SC)_{mu q} + C_{mu q} (SC)_{mu p}].
+ * orbs_occ [norb][nocc], orbs_virt [norb][nvirt], S [norb][norb] row-major
+ * device arrays; orbs_per_atom [nat] (int32) is a HOST array, like the numpy
+ * array f2py receives.  Outputs in the reference's logical order
+ * q_oo[A][i][j], q_ov[A][i][a], q_vv[A][a][b] (any output may be NULL).
+ * workspace >= dftb_trans_charges_workspace(norb, nocc, nvirt, nat) bytes. */
+size_t dftb_trans_charges_workspace(int norb, int nocc, int nvirt, int nat);
+int dftb_trans_charges(const double* orbs_occ, const double* orbs_virt, const double* S,
+                       const int* orbs_per_atom, int nat, int norb, int nocc, int nvirt,
+                       double* q_oo, double* q_ov, double* q_vv,
+                       void* workspace, size_t ws_bytes, void* stream);
+
+/* Same charges written in the engine's padded layouts (NULL = skip):
+ *   q_oo_e [nocc][nat][opad], q_ov_e [nocc][nat][vpad], q_vv_e [vpad][nat][vpad],
+ *   q_c    [nat][opad][vpad]  (the Coulomb layout).
+ * Padding entries are written as zeros. */
+int dftb_trans_charges_engine(const double* orbs_occ, const double* orbs_virt, const double* S,
+                              const int* orbs_per_atom, int nat, int norb, int nocc, int nvirt,
+                              double* q_oo_e, double* q_ov_e, double* q_vv_e, double* q_c,
+                              void* workspace, size_t ws_bytes, void* stream);
+
+/* ---- (b) Casida matvecs ---------------------------------------------------
+ * The operator of Solver.py:583-635 (_ApBv/_AmBv, Fortran twins apbv/ambv at
+ * tddftb.f90:114-287) with gamma_lr folded into q_ov and q_vv once:
+ *   (A+B)v = omega*v + 4 q_ov^T gamma q_ov v - K1 v - K2 v
+ *   (A-B)v = omega*v                          - K1 v + K2 v
+ *   (K1 v)_ia = sum_{A,j} q^{ij}_A sum_b v_jb qf^{ba}_A ,  qf = gamma_lr . q_vv
+ *   (K2 v)_ia = sum_{A,b} q^{ib}_A sum_j v_jb qf^{ja}_A ,  qf = gamma_lr . q_ov
+ * Both products are formed for the same block of vectors, so the exchange
+ * work is shared between them.  precision: 0 FP64, 1 FP32, 2 BF16x3 split
+ * pairs (FP32 accumulation); results are always FP64.  Operand pointers are
+ * FP64 (precision 0), FP32 (1) or (hi, lo) bf16 pairs (2; q_c stays FP32).
+ * For precision 2 the folded charges are atom-major (K-contiguous for the NT
+ * bf16 GEMM): qf_ov [nat][vpad][opad] and qf_vv [nat][vpad][vpad], as written
+ * by dftb_convert_bf16pair_atom_major. */
+typedef struct CasidaOperator {
+  int nat, nocc, nvirt, opad, vpad;
+  int lc;          /* 1: long-range exchange terms on */
+  int precision;   /* 0 fp64, 1 fp32, 2 bf16x3 */
+  int atom_chunk;  /* atoms per exchange slice (0 = auto) */
+  const double* omega;  /* [opad*vpad], padding 1.0 */
+  const double* gamma;  /* [nat*nat] */
+  const void* q_c;      /* [nat][opad][vpad]  f64 (precision 0) or f32 */
+  const void* q_oo[2];  /* [nocc][nat][opad] */
+  const void* q_ov[2];  /* [nocc][nat][vpad] */
+  const void* qf_ov[2]; /* [nocc][nat][vpad] gamma_lr folded */
+  const void* qf_vv[2]; /* [vpad][nat][vpad] gamma_lr folded */
+} CasidaOperator;
+
+size_t dftb_casida_matvec_workspace(const CasidaOperator* op, int nvec);
+int dftb_casida_matvec(const CasidaOperator* op, const double* V, int nvec,
+                       double* ApBV, double* AmBV, void* workspace, size_t ws_bytes, void* stream);
+
+/* Hermitian operator of HermitianDavidson.Rv (Solver.py:431-444), used when
+ * long_range_correction=0: u = om*v + 4 sqrt(omega) q^T gamma q (sqrt(omega) v),
+ * om = omega^2 + 2 omega*shift.  sqrt_omega and om are [opad*vpad]. */
+size_t dftb_hermitian_matvec_workspace(const CasidaOperator* op, int nvec);
+int dftb_hermitian_matvec(const CasidaOperator* op, const double* sqrt_omega, const double* om,
+                          const double* V, int nvec, double* RV,
+                          void* workspace, size_t ws_bytes, void* stream);
+
+/* Fold: qf[p][A][q] = sum_B gamma[A][B] q[p][B][q] for p < np, q < qpad
+ * (the gamma_lr application of Solver.py:598-604 moved out of the matvec). */
+int dftb_fold_gamma(const double* gamma, int nat, const double* q, long np, long qpad, double* qf,
+                    void* stream);
+
+/* Conversions of operands: FP64 -> FP32, FP64 -> (hi, lo) bf16 pairs. */
+int dftb_convert_f32(const double* x, long n, float* y, void* stream);
+int dftb_convert_bf16pair(const double* x, long n, unsigned short* hi, unsigned short* lo, void* stream);
+/* q[p][A][x] (p < np, x < qpad) -> (hi, lo)[A][x][p] with p zero-padded to nppad. */
+int dftb_convert_bf16pair_atom_major(const double* q, long np, long nppad, int nat, long qpad,
+                                     unsigned short* hi, unsigned short* lo, void* stream);
+
+/* ---- (c) Davidson support -------------------------------------------------
+ * Residuals and preconditioner of Solver.py:809-884 / 480-530.
+ *   W[j] = P[j] - w[j] X[j];  norms[j] = ||W[j]||_2            (j < k, length n)
+ *   Q[t] = W[sel[t]] / (w[sel[t]] - omega) with |w - omega| < 1e-6 replaced by
+ *          1e-6 * omega (Solver.py:865-868)                     (t < nsel)
+ * w, sel and norms live in device memory. */
+int dftb_residual(const double* P, const double* X, const double* w, int k, long n,
+                  double* W, double* norms, void* stream);
+int dftb_precondition(const double* W, const double* w, const int* sel, int nsel,
+                      const double* omega, long n, double* Q, void* stream);
+int dftb_row_norms(const double* X, int k, long n, double* norms, void* stream);
+/* X[j] *= s[j] */
+int dftb_scale_rows(double* X, const double* s, int k, long n, void* stream);
+
+/* Orthonormalise the m rows of Q against the l orthonormal rows of basis
+ * (two passes of classical Gram-Schmidt) and among themselves (two passes of
+ * Cholesky-QR with a rank tolerance).  Rows found linearly dependent
+ * (relative norm < tol after projection) are flagged keep[t] = 0 and zeroed.
+ * Replaces the economic QR of [bs | Q] at Solver.py:893-897. */
+size_t dftb_orthonormalize_workspace(int l, int m, long n);
+int dftb_orthonormalize(const double* basis, int l, double* Q, int m, long n, double tol,
+                        int* keep, void* workspace, size_t ws_bytes, void* stream);
+
+/* Subspace step of nonHermitianDavidson (Solver.py:759-798) for the lowest
+ * nev roots: Mmsq = sqrtm(Mm), Mh = Mmsq Mp Mmsq, Mh Tb = Tb diag(w^2),
+ * Rb = Mmsq Tb / sqrt(w), Lb = Mp Rb / w.  Mp, Mm: [l][ld] (device).
+ * Outputs w [nev], Tb, Rb, Lb [l][nev] (column n = root n).  info[0] = 0 or
+ * DFTB_ERR_NOT_SPD (Mm not positive definite or w^2 <= 0); diag[0] =
+ * sum |Mh - Mh^T|, diag[1] = sum |Lb^T Rb - 1| (the two ExcitedStatesError
+ * checks of Solver.py:766-770 and 795-798). */
+size_t dftb_subspace_workspace(int l, int nev);
+int dftb_subspace_nonhermitian(const double* Mp, const double* Mm, int l, int ld, int nev,
+                               double* w, double* Tb, double* Rb, double* Lb, int* info, double* diag,
+                               void* workspace, size_t ws_bytes, void* stream);
+/* Symmetric square root and inverse square root of an SPD matrix (coupled
+ * Newton-Schulz iteration on MFMA GEMMs; replaces scipy.linalg.sqrtm). */
+size_t dftb_sqrtm_workspace(int l);
+int dftb_sqrtm(const double* A, int l, int ld, double* Asq, double* Ainvsq, int* iters, double* err,
+               void* workspace, size_t ws_bytes, void* stream);
+/* Hermitian subspace step (Solver.py:482-484): H T = T diag(w2), lowest nev. */
+int dftb_subspace_hermitian(const double* H, int l, int ld, int nev, double* w2, double* T,
+                            int* info, void* workspace, size_t ws_bytes, void* stream);
+/* Symmetric eigensolver used by both (lowest nev pairs, ascending):
+ * Householder tridiagonalisation, bisection, inverse iteration. */
+size_t dftb_syevx_workspace(int l, int nev);
+int dftb_syevx(const double* A, int l, int ld, int nev, double* evals, double* evecs,
+               void* workspace, size_t ws_bytes, void* stream);
+
+/* out[j][e] = X[j][e] * rs[j] * cs[e]  (rs or cs may be NULL): the X+-Y
+ * scalings of Solver.py:561-568. */
+int dftb_scale2(const double* X, const double* rs, const double* cs, int k, long n, double* out, void* stream);
+/* s[j] = sum_e X[j][e]^2 wgt[e]: Lambda2 diagnostic (LR_TDDFTB.py:561-565) and
+ * the Lambda2 ordering of Solver.py:348-375. */
+int dftb_weighted_sumsq(const double* X, const double* wgt, int k, long n, double* s, void* stream);
+/* O[i][a] /= sqrt(Oii[i] Oaa[a]) on the padded grid (LR_TDDFTB.py:538-560). */
+int dftb_oia_normalize(double* O, const double* Oii, const double* Oaa, int nocc, int nvirt, int opad, int vpad,
+                       void* stream);
+
+/* Unit guess vectors at the given flat indices (Solver.py:232-251). */
+int dftb_unit_vectors(const long* idx, int k, long n, double* X, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

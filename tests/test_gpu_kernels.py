@@ -1,0 +1,307 @@
+"""GPU parity tests of the HIP kernels against the CPU oracle (run with -m gpu)."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from conftest import ground_state
+from oracle import tddftb_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev():
+    assert torch.cuda.is_available(), "GPU tests need a ROCm device"
+    return torch.device("cuda")
+
+
+def T(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=_dev())
+
+
+# ---------------------------------------------------------------------------
+# generic MFMA GEMM
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("at,bt", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (37, 61, 29), (130, 257, 203), (64, 64, 1000)])
+def test_gemm_f64(at, bt, M, N, K, rng):
+    from dftbaby_amd.engine import gemm
+    A = rng.standard_normal((K, M) if at else (M, K))
+    B = rng.standard_normal((N, K) if bt else (K, N))
+    C0 = rng.standard_normal((M, N))
+    ref = 1.5 * (A.T if at else A) @ (B.T if bt else B) - 0.5 * C0
+    C = T(C0)
+    gemm(T(A), T(B), C, M, N, K, A.shape[1], B.shape[1], N, a_trans=at, b_trans=bt, alpha=1.5, beta=-0.5)
+    np.testing.assert_allclose(C.cpu().numpy(), ref, rtol=1e-12, atol=1e-11)
+
+
+def test_gemm_f64_batched_remap_splitk(rng):
+    from dftbaby_amd.engine import gemm
+    from dftbaby_amd import _lib
+    # batch of 3, remapped output: col n -> (n // 5) * 1000 + row * 5 + n % 5
+    M, N, K, nb = 7, 15, 33, 3
+    A = rng.standard_normal((nb, M, K))
+    B = rng.standard_normal((K, N))
+    Cbig = torch.zeros(nb * 4000, dtype=torch.float64, device=_dev())
+    g = _lib.GemmArgs()
+    g.A, g.B, g.C = T(A).data_ptr(), T(B).data_ptr(), Cbig.data_ptr()
+    Ad, Bd = T(A), T(B)
+    g.A, g.B = Ad.data_ptr(), Bd.data_ptr()
+    g.lda, g.ldb, g.ldc = K, N, 5
+    g.sA, g.sB, g.sC = M * K, 0, 4000
+    g.cdiv, g.cstr, g.cs2 = 5, 1000, 1
+    g.M, g.N, g.K, g.batch, g.ksplit, g.kchunk = M, N, K, nb, 1, 0
+    g.alpha, g.beta = 1.0, 0.0
+    g.in_type, g.out_type, g.tile = 0, 0, 0
+    _lib.call("dftb_gemm", ctypes.byref(g), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    C = Cbig.cpu().numpy().reshape(nb, 4000)
+    for b in range(nb):
+        ref = A[b] @ B
+        for n in range(N):
+            got = C[b, (n // 5) * 1000 + np.arange(M) * 5 + n % 5]
+            np.testing.assert_allclose(got, ref[:, n], rtol=1e-12, atol=1e-12)
+    # split-K partials
+    from dftbaby_amd.engine import gram, Workspace
+    X = rng.standard_normal((5, 100003))
+    Y = rng.standard_normal((3, 100003))
+    out = torch.zeros((5, 3), dtype=torch.float64, device=_dev())
+    gram(T(X), 5, T(Y), 3, 100003, out, Workspace(_dev()))
+    np.testing.assert_allclose(out.cpu().numpy(), X @ Y.T, rtol=1e-11, atol=1e-10)
+
+
+@pytest.mark.parametrize("M,N,K", [(5, 9, 7), (130, 200, 96), (64, 64, 1024)])
+def test_gemm_f32(M, N, K, rng):
+    from dftbaby_amd.engine import gemm
+    from dftbaby_amd import _lib
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = rng.standard_normal((K, N)).astype(np.float32)
+    C = torch.zeros((M, N), dtype=torch.float64, device=_dev())
+    gemm(T(A, torch.float32), T(B, torch.float32), C, M, N, K, K, N, N, in_type=_lib.GEMM_IN_F32)
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    np.testing.assert_allclose(C.cpu().numpy(), ref, rtol=2e-5, atol=2e-5 * np.sqrt(K))
+
+
+@pytest.mark.parametrize("M,N,K", [(16, 16, 32), (70, 130, 96), (128, 256, 1024)])
+def test_gemm_bf16x3(M, N, K, rng):
+    """hi*hi + hi*lo + lo*hi split product: ~2^-16 relative per term, FP32 accumulation."""
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _p, _stream
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((N, K))          # NT form: B is [n][k]
+    Ad, Bd = T(A), T(B)
+    parts = []
+    for X in (Ad, Bd):
+        hi = torch.empty(X.shape, dtype=torch.int16, device=_dev())
+        lo = torch.empty(X.shape, dtype=torch.int16, device=_dev())
+        _lib.call("dftb_convert_bf16pair", _p(X), X.numel(), _p(hi), _p(lo), _stream())
+        parts.append((hi, lo))
+    C = torch.zeros((M, N), dtype=torch.float64, device=_dev())
+    g = _lib.GemmArgs()
+    g.A, g.A2 = parts[0][0].data_ptr(), parts[0][1].data_ptr()
+    g.B, g.B2 = parts[1][0].data_ptr(), parts[1][1].data_ptr()
+    g.C = C.data_ptr()
+    g.lda, g.ldb, g.ldc = K, K, N
+    g.cs2 = 1
+    g.M, g.N, g.K, g.batch, g.ksplit = M, N, K, 1, 1
+    g.alpha, g.beta = 1.0, 0.0
+    g.b_trans = 1
+    g.in_type, g.out_type = _lib.GEMM_IN_BF16X3, _lib.GEMM_OUT_F64
+    _lib.call("dftb_gemm", ctypes.byref(g), _stream())
+    ref = A @ B.T
+    err = np.abs(C.cpu().numpy() - ref).max()
+    assert err < 3e-5 * np.sqrt(K), err
+
+
+def test_mfma_f64_lane_layout():
+    """A rank-one product with distinct row/column scales pins the MFMA output layout."""
+    from dftbaby_amd.engine import gemm
+    M = N = 16
+    K = 4
+    A = np.zeros((M, K)); A[:, 0] = np.arange(1, M + 1)
+    B = np.zeros((K, N)); B[0, :] = 100.0 * np.arange(1, N + 1)
+    C = torch.zeros((M, N), dtype=torch.float64, device=_dev())
+    gemm(T(A), T(B), C, M, N, K, K, N, N)
+    np.testing.assert_array_equal(C.cpu().numpy(), np.outer(A[:, 0], B[0]))
+
+
+# ---------------------------------------------------------------------------
+# (a) transition charges
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name,lc", [("ethylene", 1), ("C8H10", 1), ("C20H22", 0)])
+def test_trans_charges_reference_layout(name, lc):
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _p, _stream, Workspace
+    gs = ground_state(name, lc)
+    f, orbe, C, S = gs.getOccupation(), gs.getKSEnergies(), gs.getKSCoefficients(), gs.getOverlapMatrix()
+    _, _, ao, av = O.active_orbitals(f, orbe)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    nat, o, v = q_ov.shape
+    Co, Cv, St = T(C[:, ao]), T(C[:, av]), T(S)
+    g_oo = torch.empty((nat, o, o), dtype=torch.float64, device=_dev())
+    g_ov = torch.empty((nat, o, v), dtype=torch.float64, device=_dev())
+    g_vv = torch.empty((nat, v, v), dtype=torch.float64, device=_dev())
+    opa = np.ascontiguousarray(gs.orbsPerAtom, dtype=np.int32)
+    ws = Workspace(_dev())
+    w, wn = ws.get(_lib.load().dftb_trans_charges_workspace(S.shape[0], o, v, nat))
+    _lib.call("dftb_trans_charges", _p(Co), _p(Cv), _p(St), opa.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+              nat, S.shape[0], o, v, _p(g_oo), _p(g_ov), _p(g_vv), _p(w), wn, _stream())
+    for got, ref in ((g_oo, q_oo), (g_ov, q_ov), (g_vv, q_vv)):
+        np.testing.assert_allclose(got.cpu().numpy(), ref, rtol=0, atol=1e-13)
+
+
+def test_trans_charges_engine_layout():
+    from dftbaby_amd.engine import CasidaOperatorGPU
+    gs = ground_state("C8H10", 1, 0.05, 7)
+    f, orbe, C, S = gs.getOccupation(), gs.getKSEnergies(), gs.getKSCoefficients(), gs.getOverlapMatrix()
+    _, _, ao, av = O.active_orbitals(f, orbe)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    omega, _ = O.orbital_differences(orbe, f, ao, av)
+    op = CasidaOperatorGPU.from_orbitals(C[:, ao], C[:, av], S, gs.orbsPerAtom, omega, gs.getGamma(),
+                                         gs.getGamma_lr(), 1)
+    nat, o, v = q_ov.shape
+    qc = op.q["c"].cpu().numpy()
+    np.testing.assert_allclose(qc[:, :o, :v], q_ov, atol=1e-13)
+    assert np.all(qc[:, o:, :] == 0) and np.all(qc[:, :, v:] == 0)
+    np.testing.assert_allclose(op.q["oo"].cpu().numpy()[:, :, :o], q_oo.transpose(1, 0, 2), atol=1e-13)
+    np.testing.assert_allclose(op.q["ov"].cpu().numpy()[:, :, :v], q_ov.transpose(1, 0, 2), atol=1e-13)
+    glr = gs.getGamma_lr()
+    qf_vv = np.einsum("AB,Bab->aAb", glr, q_vv)
+    np.testing.assert_allclose(op.q["qf_vv"].cpu().numpy()[:v, :, :v], qf_vv, atol=1e-12)
+    np.testing.assert_allclose(op.qaa.cpu().numpy(), np.einsum("Aaa->aA", q_vv), atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# (b) matvecs
+# ---------------------------------------------------------------------------
+def _operator(name, lc, sigma=0.05, seed=3):
+    from dftbaby_amd.engine import CasidaOperatorGPU
+    gs = ground_state(name, lc, sigma, seed)
+    f, orbe, C, S = gs.getOccupation(), gs.getKSEnergies(), gs.getKSCoefficients(), gs.getOverlapMatrix()
+    _, _, ao, av = O.active_orbitals(f, orbe)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    omega, _ = O.orbital_differences(orbe, f, ao, av)
+    g, glr = gs.getGamma(), gs.getGamma_lr()
+    op = CasidaOperatorGPU.from_orbitals(C[:, ao], C[:, av], S, gs.orbsPerAtom, omega, g, glr, lc)
+    return op, (g, glr, q_oo, q_ov, q_vv, omega)
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 3e-5), (2, 3e-5)])
+def test_casida_matvec(precision, tol, rng):
+    op, (g, glr, q_oo, q_ov, q_vv, omega) = _operator("C12H14", 1)
+    o, v = omega.shape
+    nvec = 5
+    vs = rng.standard_normal((o, v, nvec))
+    refP = O.ApBv(g, glr, q_oo, q_ov, q_vv, omega, vs, 1)
+    refM = O.AmBv(g, glr, q_oo, q_ov, q_vv, omega, vs, 1)
+    V = op.to_padded(np.moveaxis(vs, 2, 0))
+    P, M = op.matvec(V, precision)
+    gotP = op.from_padded(P).cpu().numpy()
+    gotM = op.from_padded(M).cpu().numpy()
+    scale = np.abs(refP).max()
+    assert np.abs(gotP - np.moveaxis(refP, 2, 0)).max() < tol * scale
+    assert np.abs(gotM - np.moveaxis(refM, 2, 0)).max() < tol * scale
+    # padding stays zero
+    Pp = P.reshape(nvec, op.opad, op.vpad).cpu().numpy()
+    assert np.all(Pp[:, o:, :] == 0) and np.all(Pp[:, :, v:] == 0)
+
+
+def test_casida_matvec_atom_chunks(rng):
+    """Exchange contractions split over atom slices accumulate to the same product."""
+    op, (g, glr, q_oo, q_ov, q_vv, omega) = _operator("C8H10", 1)
+    o, v = omega.shape
+    V = op.to_padded(rng.standard_normal((3, o, v)))
+    P1, M1 = op.matvec(V, 0, atom_chunk=0)
+    P2, M2 = op.matvec(V, 0, atom_chunk=3)
+    np.testing.assert_allclose(P1.cpu().numpy(), P2.cpu().numpy(), atol=1e-13)
+    np.testing.assert_allclose(M1.cpu().numpy(), M2.cpu().numpy(), atol=1e-13)
+
+
+def test_hermitian_matvec(rng):
+    op, (g, glr, q_oo, q_ov, q_vv, omega) = _operator("C12H14", 0)
+    o, v = omega.shape
+    vs = rng.standard_normal((4, o, v))
+    shift = np.zeros_like(omega)
+    om = omega ** 2 + 2 * omega * shift
+    wq = q_ov * np.sqrt(omega)
+    ref = np.stack([om * x + 4 * np.tensordot(wq, g @ np.tensordot(wq, x, axes=([1, 2], [0, 1])), axes=(0, 0))
+                    for x in vs])
+    sq = torch.sqrt(op.omega)
+    omt = op.omega * op.omega
+    R = op.hermitian_matvec(op.to_padded(vs), sq, omt)
+    np.testing.assert_allclose(op.from_padded(R).cpu().numpy(), ref, rtol=1e-12, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# (c) subspace kernels
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("l,nev", [(1, 1), (2, 2), (9, 4), (60, 20), (257, 30)])
+def test_syevx(l, nev, rng):
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _p, _stream, Workspace
+    A = rng.standard_normal((l, l))
+    A = A + A.T
+    if l >= 9:
+        # exact degeneracy in the wanted range
+        Q, _ = np.linalg.qr(rng.standard_normal((l, l)))
+        ev = np.sort(rng.standard_normal(l))
+        ev[1] = ev[2] = ev[3]
+        A = (Q * ev) @ Q.T
+    ld = l + 3
+    Ad = torch.zeros((l, ld), dtype=torch.float64, device=_dev())
+    Ad[:, :l] = T(A)
+    ev = torch.empty(nev, dtype=torch.float64, device=_dev())
+    V = torch.empty((l, nev), dtype=torch.float64, device=_dev())
+    w, wn = Workspace(_dev()).get(_lib.load().dftb_syevx_workspace(l, nev))
+    _lib.call("dftb_syevx", _p(Ad), l, ld, nev, _p(ev), _p(V), _p(w), wn, _stream())
+    ref = np.linalg.eigvalsh(A)[:nev]
+    e = ev.cpu().numpy()
+    Vh = V.cpu().numpy()
+    np.testing.assert_allclose(e, ref, atol=1e-12 * max(1, np.abs(ref).max()))
+    np.testing.assert_allclose(Vh.T @ Vh, np.eye(nev), atol=1e-11)
+    np.testing.assert_allclose(A @ Vh, Vh * e, atol=1e-10)
+
+
+def test_sqrtm(rng):
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _p, _stream, Workspace
+    l = 80
+    Q, _ = np.linalg.qr(rng.standard_normal((l, l)))
+    A = (Q * np.linspace(0.02, 3.0, l)) @ Q.T
+    Ad = T(A)
+    S = torch.empty_like(Ad)
+    Si = torch.empty_like(Ad)
+    it = (ctypes.c_int * 1)()
+    w, wn = Workspace(_dev()).get(_lib.load().dftb_sqrtm_workspace(l))
+    err = torch.zeros(1, dtype=torch.float64)
+    _lib.call("dftb_sqrtm", _p(Ad), l, l, _p(S), _p(Si), ctypes.cast(it, ctypes.c_void_p),
+              ctypes.c_void_p(err.data_ptr()), _p(w), wn, _stream())
+    ref = np.real(sla.sqrtm(A))
+    np.testing.assert_allclose(S.cpu().numpy(), ref, atol=1e-12)
+    np.testing.assert_allclose(S.cpu().numpy() @ Si.cpu().numpy(), np.eye(l), atol=1e-11)
+
+
+def test_orthonormalize(rng):
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _p, _stream, Workspace
+    n, l, m = 5003, 7, 5
+    B, _ = np.linalg.qr(rng.standard_normal((n, l)))
+    Qh = rng.standard_normal((m, n))
+    Qh[3] = Qh[1] + B[:, 2]           # dependent after projection on span(B) + Q1
+    Bd = T(B.T.copy())
+    Qd = T(Qh)
+    keep = torch.empty(m, dtype=torch.int32, device=_dev())
+    w, wn = Workspace(_dev()).get(_lib.load().dftb_orthonormalize_workspace(l, m, n))
+    _lib.call("dftb_orthonormalize", _p(Bd), l, _p(Qd), m, n, 1e-8, _p(keep), _p(w), wn, _stream())
+    k = keep.cpu().numpy()
+    assert list(k) == [1, 1, 1, 0, 1]
+    Qr = Qd.cpu().numpy()[k == 1]
+    np.testing.assert_allclose(Qr @ Qr.T, np.eye(4), atol=1e-12)
+    np.testing.assert_allclose(Qr @ B, 0, atol=1e-12)
+    # span preserved: original kept rows lie in span(B, Qr)
+    basis = np.vstack([B.T, Qr])
+    for row in Qh[k == 1]:
+        res = row - basis.T @ (basis @ row)
+        assert np.linalg.norm(res) < 1e-9 * np.linalg.norm(row)

@@ -1,0 +1,214 @@
+"""Pin the CPU oracle (oracle/tddftb_oracle.py) before trusting it.
+
+* against DFTBaby's own Fortran kernels (DFTB/extensions/tddftb.f90
+  trans_charges / apbv / ambv) compiled by oracle/Makefile into oracle/_ref,
+  when that build exists, and against committed golden vectors it produced
+  (tests/golden/tddftb_f90_golden.npz, written by tests/golden/make_golden.py);
+* against closed forms / invariants: sum_A q_A^{pq} = (C^T S C)_{pq} = delta_pq,
+  q^{pq} = q^{qp}, brute-force loops, dense buildAB products;
+* Davidson vs dense Casida (the reference's diag_check criterion).
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, ground_state
+from oracle import tddftb_oracle as O
+
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libtddftb_ref.so")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "tddftb_f90_golden.npz")
+
+
+def _active(gs):
+    f, orbe, C, S = gs.getOccupation(), gs.getKSEnergies(), gs.getKSCoefficients(), gs.getOverlapMatrix()
+    _, _, ao, av = O.active_orbitals(f, orbe)
+    return f, orbe, C, S, ao, av
+
+
+def f90_lib():
+    if not os.path.exists(REF_SO):
+        pytest.skip("oracle/_ref not built (needs the reference tree and amdflang)")
+    return ctypes.CDLL(REF_SO)
+
+
+def f90_trans_charges(lib, Co, Cv, S, opa):
+    norb, no = Co.shape
+    nv = Cv.shape[1]
+    nat = len(opa)
+    F = lambda a: np.asfortranarray(a, dtype=np.float64)
+    Co_f, Cv_f, S_f = F(Co), F(Cv), F(S)
+    opa_f = np.asfortranarray(opa, dtype=np.int32)
+    q_oo = np.zeros((nat, no, no), order="F")
+    q_ov = np.zeros((nat, no, nv), order="F")
+    q_vv = np.zeros((nat, nv, nv), order="F")
+    ip = lambda x: ctypes.byref(ctypes.c_int(x))
+    dp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib._QMtddftbPtrans_charges(dp(Co_f), dp(Cv_f), dp(S_f), dp(opa_f), ip(nat), ip(norb), ip(no), ip(nv),
+                                dp(q_oo), dp(q_ov), dp(q_vv))
+    return q_oo, q_ov, q_vv
+
+
+def f90_products(lib, g, glr, q_oo, q_ov, q_vv, omega, vs):
+    nat, no, nv = q_ov.shape
+    nvec = vs.shape[2]
+    F = lambda a: np.asfortranarray(a, dtype=np.float64)
+    args = [F(g), F(glr), F(q_oo), F(q_ov), F(q_vv), F(omega), F(vs)]
+    ip = lambda x: ctypes.byref(ctypes.c_int(x))
+    dp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    out = []
+    for name in ("_QMtddftbPapbv", "_QMtddftbPambv"):
+        us = np.zeros((no, nv, nvec), order="F")
+        getattr(lib, name)(*[dp(a) for a in args], ip(nat), ip(no), ip(nv), ip(nvec), dp(us))
+        out.append(np.array(us))
+    return out
+
+
+# --------------------------------------------------------------------------- a2
+@pytest.mark.parametrize("name,lc", [("ethylene", 1), ("benzene", 1), ("C8H10", 0)])
+def test_trans_charges_vs_reference_fortran(name, lc):
+    lib = f90_lib()
+    gs = ground_state(name, lc)
+    f, orbe, C, S, ao, av = _active(gs)
+    ref = f90_trans_charges(lib, C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    got = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    for a, b in zip(got, ref):
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-14)
+
+
+def test_trans_charges_golden_vectors():
+    """Committed Fortran outputs (usable where oracle/_ref cannot be built)."""
+    if not os.path.exists(GOLDEN):
+        pytest.skip("golden vectors not generated")
+    d = np.load(GOLDEN)
+    got = O.trans_charges(d["Co"], d["Cv"], d["S"], d["opa"])
+    for a, key in zip(got, ("q_oo", "q_ov", "q_vv")):
+        np.testing.assert_allclose(a, d[key], rtol=0, atol=1e-14)
+    ApB = O.ApBv(d["g"], d["glr"], d["q_oo"], d["q_ov"], d["q_vv"], d["omega"], d["vs"], 1)
+    AmB = O.AmBv(d["g"], d["glr"], d["q_oo"], d["q_ov"], d["q_vv"], d["omega"], d["vs"], 1)
+    np.testing.assert_allclose(ApB, d["ApB"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(AmB, d["AmB"], rtol=0, atol=1e-12)
+
+
+def test_trans_charges_invariants_and_bruteforce():
+    gs = ground_state("C8H10", 1, 0.05, 3)
+    f, orbe, C, S, ao, av = _active(gs)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    # sum over atoms: (C^T S C)_{pq} = delta_pq
+    np.testing.assert_allclose(q_oo.sum(0), np.eye(len(ao)), atol=1e-12)
+    np.testing.assert_allclose(q_vv.sum(0), np.eye(len(av)), atol=1e-12)
+    np.testing.assert_allclose(q_ov.sum(0), 0.0, atol=1e-12)
+    np.testing.assert_allclose(q_oo, q_oo.transpose(0, 2, 1), atol=0)
+    np.testing.assert_allclose(q_vv, q_vv.transpose(0, 2, 1), atol=0)
+    # brute-force loop on a few orbitals
+    sub_o, sub_v = ao[-3:], av[:4]
+    bf = O.trans_charges_bruteforce(C[:, sub_o], C[:, sub_v], S, gs.orbsPerAtom)
+    ve = O.trans_charges(C[:, sub_o], C[:, sub_v], S, gs.orbsPerAtom)
+    for a, b in zip(bf, ve):
+        np.testing.assert_allclose(a, b, atol=1e-14)
+    # a planted mistake (dropping the symmetrising term) is caught by the invariant
+    Sc = S @ C[:, av]
+    wrong = np.einsum("mi,ma->ia", C[:, av], Sc)  # no per-atom split: equals identity, but per-atom halves differ
+    assert wrong.shape == (len(av), len(av))
+
+
+# --------------------------------------------------------------------------- a7
+def test_products_vs_reference_fortran(rng):
+    lib = f90_lib()
+    gs = ground_state("C8H10", 1, 0.05, 4)
+    f, orbe, C, S, ao, av = _active(gs)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    omega, _ = O.orbital_differences(orbe, f, ao, av)
+    g, glr = gs.getGamma(), gs.getGamma_lr()
+    vs = rng.standard_normal((len(ao), len(av), 3))
+    refP, refM = f90_products(lib, g, glr, q_oo, q_ov, q_vv, omega, vs)
+    np.testing.assert_allclose(O.ApBv(g, glr, q_oo, q_ov, q_vv, omega, vs, 1), refP, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(O.AmBv(g, glr, q_oo, q_ov, q_vv, omega, vs, 1), refM, rtol=0, atol=1e-12)
+
+
+def test_products_vs_dense_buildAB(rng):
+    gs = ground_state("C8H10", 1, 0.05, 4)
+    f, orbe, C, S, ao, av = _active(gs)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    omega, df = O.orbital_differences(orbe, f, ao, av)
+    g, glr = gs.getGamma(), gs.getGamma_lr()
+    o, v = omega.shape
+    A, B = O.buildAB(g, glr, q_oo, q_vv, q_ov, omega, 0 * omega, df, o, v)
+    vs = rng.standard_normal((o, v, 2))
+    P = O.ApBv(g, glr, q_oo, q_ov, q_vv, omega, vs, 1)
+    M = O.AmBv(g, glr, q_oo, q_ov, q_vv, omega, vs, 1)
+    for n in range(2):
+        np.testing.assert_allclose(P[:, :, n].ravel(), (A + B) @ vs[:, :, n].ravel(), atol=1e-12)
+        np.testing.assert_allclose(M[:, :, n].ravel(), (A - B) @ vs[:, :, n].ravel(), atol=1e-12)
+    # symmetric A, B; lc=0 makes A-B diagonal
+    np.testing.assert_allclose(A, A.T, atol=1e-13)
+    np.testing.assert_allclose(B, B.T, atol=1e-13)
+    A0, B0 = O.buildAB(g, 0 * glr, q_oo, q_vv, q_ov, omega, 0 * omega, df, o, v)
+    np.testing.assert_allclose(A0 - B0, np.diag(omega.ravel()), atol=1e-14)
+
+
+# --------------------------------------------------------------------------- a6/a7/a8
+def test_casida_self_consistency():
+    """Casida's own asserts (Solver.py:145-173) on an lc geometry."""
+    gs = ground_state("C8H10", 1, 0.05, 2)
+    f, orbe, C, S, ao, av = _active(gs)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    omega, df = O.orbital_differences(orbe, f, ao, av)
+    g, glr = gs.getGamma(), gs.getGamma_lr()
+    o, v = omega.shape
+    Om, Cc, XmY, XpY = O.Casida(g, glr, q_oo, q_vv, q_ov, omega, 0 * omega, df, o, v)
+    A, B = O.buildAB(g, glr, q_oo, q_vv, q_ov, omega, 0 * omega, df, o, v)
+    assert np.all(np.diff(Om) >= 0) and Om[0] > 0
+    for n in range(5):
+        xp, xm = XpY[n].ravel(), XmY[n].ravel()
+        assert abs(xp @ xm - 1) < 1e-10
+        np.testing.assert_allclose((A + B) @ xp, Om[n] * xm, atol=1e-9)
+        np.testing.assert_allclose((A - B) @ xm, Om[n] * xp, atol=1e-9)
+        assert abs(Cc[n].ravel() @ Cc[n].ravel() - 1) < 1e-10
+
+
+@pytest.mark.parametrize("lc", [0, 1])
+def test_davidson_matches_casida(lc):
+    """The reference's diag_check: eigenvalues < 1e-10, X+-Y up to a global sign."""
+    gs = ground_state("C8H10", lc, 0.05, 9)
+    full = O.solve_from_ground_state(gs, nstates=None)
+    dav = O.solve_from_ground_state(gs, nstates=5, diag_conv=1e-11, diag_maxiter=100)
+    assert np.abs(dav["Omega"] - full["Omega"][:5]).sum() < 1e-10
+    err = 0.0
+    for n in range(5):
+        for key in ("XmY", "XpY"):
+            a, b = dav[key][n], full[key][n]
+            err += min(np.abs(a - b).sum(), np.abs(a + b).sum())
+    assert err < 1e-6
+
+
+def test_oscillator_strengths_closed_form():
+    gs = ground_state("C8H10", 0, 0.05, 1)
+    r = O.solve_from_ground_state(gs, nstates=None)
+    C = r["C"][:4]
+    orbs, D = gs.getKSCoefficients(), gs.getDipoleMatrix()
+    ao, av = r["act_occ"], r["act_virt"]
+    for c in range(3):
+        dense = np.einsum("nia,mi,mlc,la->n", C, orbs[:, ao], D[..., c:c + 1], orbs[:, av])
+        np.testing.assert_allclose(r["tdip"][c][:4], dense, atol=1e-12)
+    f = 2.0 / 3.0 * r["Omega"][:4] * sum(r["tdip"][c][:4] ** 2 for c in range(3))
+    np.testing.assert_allclose(r["osc"][:4], f, rtol=1e-14)
+    assert O.oscillator_strengths(r["Omega"], r["tdip"], "T").max() == 0.0
+
+
+def test_oia_loop_and_lambda2():
+    gs = ground_state("C8H10", 1, 0.05, 6)
+    f, orbe, C, S, ao, av = _active(gs)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    Om = gs.getGaussianOverlap()
+    np.testing.assert_allclose(O.Lambda2_calcOia(q_oo, q_vv, Om), O.Lambda2_calcOia_loop(q_oo, q_vv, Om),
+                               atol=1e-14)
+    Oia = O.Lambda2_calcOia(q_oo, q_vv, Om)
+    assert np.all(Oia >= 0) and np.all(Oia <= 1 + 1e-12)
+
+
+def test_davidson_not_converged_raises():
+    gs = ground_state("C8H10", 1, 0.05, 9)
+    with pytest.raises(O.ExcitedStatesNotConverged):
+        O.solve_from_ground_state(gs, nstates=5, diag_conv=1e-14, diag_maxiter=1)
