@@ -79,6 +79,7 @@ _SIGS = {
     "dftb_syevx_workspace": (_sz, [_i, _i]),
     "dftb_syevx": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _sz, _vp]),
     "dftb_unit_vectors": (_i, [_vp, _i, _l, _vp, _vp]),
+    "dftb_symmetrize": (_i, [_vp, _i, _i, _vp, _vp]),
 }
 
 EXPORTED = sorted(_SIGS)

@@ -151,7 +151,9 @@ __global__ __launch_bounds__(256) void cholqr_kernel(const double* __restrict__ 
       double d = G[j * m + j];
       for (int k = 0; k < j; ++k) d -= L[j * m + k] * L[j * m + k];
       double r = ref ? ref[j] * ref[j] : 1.0;
-      int ok = keep[j] && (d > tol2 * r) && (d > 0.0);
+      // the pivot of a dependent row is Cholesky cancellation noise ~ eps * G_jj:
+      // require it to clear both the caller's tolerance and 1e-12 * G_jj
+      int ok = keep[j] && (d > tol2 * r) && (d > 1.0e-12 * G[j * m + j]) && (d > 0.0);
       s_keep = ok;
       keep[j] = ok;
       L[j * m + j] = ok ? sqrt(d) : 1.0;

@@ -596,3 +596,46 @@ extern "C" int dftb_subspace_hermitian(const double* H, int l, int ld, int nev, 
   DFTB_CHECK_LAUNCH();
   return dftb_syevx(H, l, ld, nev, w2, T, (char*)workspace + S.eig, eig_layout(l, nev).total, stream);
 }
+
+// Replace the leading l x l block of M by its symmetric part; asym[0] = max |M - M^T| / max |M|.
+__global__ __launch_bounds__(1024) void symmetrize_kernel(double* M, int l, int ld, double* asym) {
+  __shared__ double sh[16], sm[16];
+  double a = 0.0, mx = 0.0;
+  for (long i = threadIdx.x; i < (long)l * l; i += blockDim.x) {
+    int r = (int)(i / l), c = (int)(i % l);
+    if (c > r) {
+      double x = M[(long)r * ld + c], y = M[(long)c * ld + r];
+      a = fmax(a, fabs(x - y));
+      mx = fmax(mx, fmax(fabs(x), fabs(y)));
+      double s = 0.5 * (x + y);
+      M[(long)r * ld + c] = s;
+      M[(long)c * ld + r] = s;
+    } else if (c == r) {
+      mx = fmax(mx, fabs(M[(long)r * ld + c]));
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a = fmax(a, __shfl_xor(a, o, 64));
+    mx = fmax(mx, __shfl_xor(mx, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[threadIdx.x >> 6] = a;
+    sm[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double A = 0.0, X = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+      A = fmax(A, sh[i]);
+      X = fmax(X, sm[i]);
+    }
+    asym[0] = X > 0.0 ? A / X : 0.0;
+  }
+}
+
+extern "C" int dftb_symmetrize(double* M, int l, int ld, double* asym, void* stream) {
+  if (l <= 0 || ld < l || !M || !asym) return DFTB_ERR_ARG;
+  hipLaunchKernelGGL(symmetrize_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, M, l, ld, asym);
+  DFTB_CHECK_LAUNCH();
+  return DFTB_OK;
+}

@@ -21,6 +21,8 @@
 #include "common.h"
 #include "gemm.h"
 
+template <typename T> struct IsF64 { static constexpr bool value = false; };
+template <> struct IsF64<double> { static constexpr bool value = true; };
 template <typename T> struct AccT;
 template <> struct AccT<double> { typedef d4_t type; };
 template <> struct AccT<float> { typedef f4_t type; };
@@ -153,7 +155,10 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs p) {
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        long row = m0 + wm * (BM / 2) + i * 16 + 4 * (lane >> 4) + r;
+        // accumulator row: v_mfma_f64_16x16x4f64 holds rows (lane/16) + 4r,
+        // v_mfma_f32_16x16x4f32 rows 4(lane/16) + r (both: column lane%16)
+        const int rr = IsF64<T>::value ? ((lane >> 4) + 4 * r) : (4 * (lane >> 4) + r);
+        long row = m0 + wm * (BM / 2) + i * 16 + rr;
         long col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
         if (row < M && col < N) store_out(p, cbase + remap_addr(p, row, col), p.alpha * (double)acc[i][j][r]);
       }

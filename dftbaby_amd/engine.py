@@ -66,9 +66,9 @@ class Workspace:
         self.buf = torch.empty(0, dtype=torch.uint8, device=device)
 
     def get(self, nbytes):
-        nbytes = int(nbytes)
+        nbytes = (int(nbytes) + 255) // 256 * 256
         if self.buf.numel() < nbytes:
-            self.buf = torch.empty(int(nbytes * 1.1) + 4096, dtype=torch.uint8, device=self.device)
+            self.buf = torch.empty((int(nbytes * 1.1) + 4096) // 256 * 256, dtype=torch.uint8, device=self.device)
         return self.buf, self.buf.numel()
 
 
@@ -106,7 +106,7 @@ def gram(X, P, Y, Q, n, out, ws, ldx=None, ldy=None, ldo=None, x_off=0, y_off=0,
     kchunk = ((n + ks - 1) // ks + 31) // 32 * 32
     ks = (n + kchunk - 1) // kchunk
     part, _ = ws.get(ks * P * Q * 8)
-    part = part.view(torch.float64)
+    part = part[:ks * P * Q * 8].view(torch.float64)
     gemm(X, Y, part, P, Q, n, ldx, ldy, Q, b_trans=1, ksplit=ks, kchunk=kchunk, sCk=P * Q, tile=64,
          A_off=x_off, B_off=y_off)
     red = part[:ks * P * Q].view(ks, P, Q).sum(0) if ks > 1 else part[:P * Q].view(P, Q)
@@ -154,7 +154,7 @@ class CasidaOperatorGPU:
         o, v, op, vp = self.nocc, self.nvirt, self.opad, self.vpad
         z = lambda *s: torch.empty(s, dtype=torch.float64, device=dev)
         q_c = z(nat, op, vp)
-        q_oo = z(o, nat, op) if self.lc else None
+        q_oo = z(o, nat, op)           # also needed (diagonal only) for Lambda2 when lc = 0
         q_ov = z(o, nat, vp) if self.lc else None
         q_vv = z(vp, nat, vp)
         nb = _lib.load().dftb_trans_charges_workspace(norb, o, v, nat)
@@ -164,10 +164,11 @@ class CasidaOperatorGPU:
              _p(q_oo), _p(q_ov), _p(q_vv), _p(q_c), _p(w), wn, _stream())
         self.q["c"] = q_c
         self._diagonals_from_engine(q_c, q_vv)
+        self.qii = q_oo[:, :, :o].diagonal(dim1=0, dim2=2).T.contiguous()   # q^{ii}_A as [o][nat]
         if self.lc:
             self.q["oo"], self.q["ov"] = q_oo, q_ov
             self._fold(q_vv)
-        del q_vv
+        del q_vv, q_oo
         return self
 
     @classmethod
@@ -384,6 +385,19 @@ def _precondition_select(norms_L, norms_R, k, l, kmax, conv):
     return sel
 
 
+def _orthonormalize_block(bs, l, m, N, ws, st):
+    """Orthonormalise rows bs[l:l+m] against bs[:l] and among themselves; compact; return new count."""
+    keep = torch.empty(m, dtype=torch.int32, device=bs.device)
+    nb = _lib.load().dftb_orthonormalize_workspace(l, m, N)
+    wsb, wsn = ws.get(nb)
+    call("dftb_orthonormalize", _p(bs), l, _p(bs[l:l + m]), m, N, 1.0e-8, _p(keep), _p(wsb), wsn, st)
+    kept = np.where(keep.cpu().numpy() != 0)[0]
+    if 0 < len(kept) < m:
+        Q = bs[l:l + m]
+        Q[:len(kept)] = Q[torch.as_tensor(kept, device=bs.device)].clone()
+    return l + len(kept)
+
+
 def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, XmYguess=None, XpYguess=None,
                               w_guess=None, precision=0, verbose=0, refine=True, stats=None):
     """Non-Hermitian Davidson on the GPU (Solver.py:637-917).
@@ -416,10 +430,12 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
         call("dftb_row_norms", _p(G), lmax, N, _p(nrm), st)
         inv = 1.0 / nrm
         call("dftb_scale2", _p(G), _p(inv), None, lmax, N, _p(bs), st)
+        lmax = _orthonormalize_block(bs, 0, lmax, N, ws, st)
     l = lmax
     new0 = 0
     info = torch.zeros(2, dtype=torch.int32, device=dev)
     diag = torch.zeros(2, dtype=torch.float64, device=dev)
+    asym = torch.zeros(1, dtype=torch.float64, device=dev)
     wdev = torch.zeros(k, dtype=torch.float64, device=dev)
     Tb = torch.zeros((cap, k), dtype=torch.float64, device=dev)
     Rb = torch.zeros((cap, k), dtype=torch.float64, device=dev)
@@ -447,6 +463,13 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
                 if verbose > 1:
                     torch.cuda.synchronize()
                 stats.matvec_time += time.time() - t0
+            if precision > 0:
+                # low-precision products: use the symmetric part of M+-, sanity-check the asymmetry
+                for Mx in (Mp, Mm):
+                    call("dftb_symmetrize", _p(Mx), l, cap, _p(asym), st)
+                    a = float(asym.cpu()[0])
+                    if a > 1.0e-3:
+                        raise ExcitedStatesError("subspace matrix not symmetric (relative asymmetry %.3e)" % a)
             nb = _lib.load().dftb_subspace_workspace(l, k)
             wsb, wsn = ws.get(nb)
             call("dftb_subspace_nonhermitian", _p(Mp), _p(Mm), l, cap, k, _p(wdev), _p(Tb), _p(Rb), _p(Lb),
@@ -554,6 +577,7 @@ def hermitian_davidson_gpu(op, nstates, omega_shift=None, ifact=1, conv=1.0e-14,
         nrm = torch.empty(lmax, dtype=torch.float64, device=dev)
         call("dftb_row_norms", _p(G), lmax, N, _p(nrm), st)
         call("dftb_scale2", _p(G), _p(1.0 / nrm), None, lmax, N, _p(bs), st)
+        lmax = _orthonormalize_block(bs, 0, lmax, N, ws, st)
     l = lmax
     new0 = 0
     info = torch.zeros(2, dtype=torch.int32, device=dev)

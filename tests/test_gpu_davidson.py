@@ -41,11 +41,17 @@ def test_nonhermitian_davidson_vs_casida(name, sigma):
     cc = op.from_padded(Cg).cpu().numpy()
     assert _sign_err(xpy, full[3][:k]) < 1e-6
     assert _sign_err(xmy, full[2][:k]) < 1e-6
-    assert _sign_err(cc, full[1][:k]) < 1e-6
-    # same states as the oracle's own Davidson, with no more iterations
+    # C = bs Tb depends on the expansion space, not only on the roots (the reference's
+    # own diag_check leaves C out, LR_TDDFTB.py:513-516): compare with the oracle's
+    # Davidson, which builds the same spans iteration by iteration.
     ref = O.nonHermitianDavidson(g, glr, q_oo, q_vv, q_ov, omega, o, v, None, None, None, nstates=k,
                                  conv=1e-10, maxiter=100)
     assert np.abs(ref[0] - w).max() < 1e-10
+    assert st.iterations == len(O.nonHermitianDavidson.history)
+    # the spans agree only up to rounding amplified in the last, nearly converged
+    # expansion vectors: C agrees to ~1e-5 per element, checked per state in 2-norm
+    for a, b in zip(cc, ref[1]):
+        assert min(np.linalg.norm(a - b), np.linalg.norm(a + b)) < 1e-4
 
 
 def test_hermitian_davidson_vs_casida_c20h22():

@@ -124,7 +124,23 @@ def test_mfma_f64_lane_layout():
     B = np.zeros((K, N)); B[0, :] = 100.0 * np.arange(1, N + 1)
     C = torch.zeros((M, N), dtype=torch.float64, device=_dev())
     gemm(T(A), T(B), C, M, N, K, K, N, N)
-    np.testing.assert_array_equal(C.cpu().numpy(), np.outer(A[:, 0], B[0]))
+    got = C.cpu().numpy()
+    print("row ids seen per output row:", np.round(got[:, 0] / 100.0).astype(int).tolist())
+    np.testing.assert_array_equal(got, np.outer(A[:, 0], B[0]))
+
+
+def test_mfma_f32_lane_layout():
+    from dftbaby_amd.engine import gemm
+    from dftbaby_amd import _lib
+    M = N = 16
+    K = 4
+    A = np.zeros((M, K), dtype=np.float32); A[:, 0] = np.arange(1, M + 1)
+    B = np.zeros((K, N), dtype=np.float32); B[0, :] = 100.0 * np.arange(1, N + 1)
+    C = torch.zeros((M, N), dtype=torch.float64, device=_dev())
+    gemm(T(A, torch.float32), T(B, torch.float32), C, M, N, K, K, N, N, in_type=_lib.GEMM_IN_F32)
+    got = C.cpu().numpy()
+    print("f32 row ids seen per output row:", np.round(got[:, 0] / 100.0).astype(int).tolist())
+    np.testing.assert_array_equal(got, np.outer(A[:, 0], B[0]))
 
 
 # ---------------------------------------------------------------------------
