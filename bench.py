@@ -31,9 +31,9 @@ HBM_PEAK = 8.0e12
 def workload_atoms(name):
     from dftbaby_amd.groundstate import geometry as g
     if name == "flake":
-        return g.displace(g.graphene_flake(24, 25), 0.02, 20261202), 20
+        return g.displace(g.graphene_flake(24, 25, prune=True), 0.02, 20261202), 20
     if name == "flake-small":
-        return g.displace(g.graphene_flake(8, 9), 0.02, 20261202), 10
+        return g.displace(g.graphene_flake(8, 9, prune=True), 0.02, 20261202), 10
     if name == "polyene20":
         return g.polyene(20), 10
     raise ValueError(name)
@@ -160,6 +160,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
 
+    from dftbaby_amd import engine
+    engine.PROGRESS = log
     atoms, nstates = workload_atoms(args.workload)
     t0 = time.time()
     gs = ground_state(atoms)

@@ -32,6 +32,11 @@ from . import _lib
 from ._lib import GemmArgs, call
 
 
+# Optional progress hook: called as PROGRESS(text) once per Davidson iteration
+# (bench.py points it at stderr so long solves show they are alive).
+PROGRESS = None
+
+
 class ExcitedStatesNotConverged(Exception):
     pass
 
@@ -505,6 +510,8 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
         nsum = nL + nR
         w_host = wdev.cpu().numpy()
         stats.history.append((l, nsum.copy(), w_host.copy()))
+        if PROGRESS is not None:
+            PROGRESS("davidson it %d: l=%d max res %.3e (%.1f s)" % (it, l, float(nsum.max()), time.time() - t_start))
         if verbose > 0:
             print("Iteration %d: ( %d expansion vectors )   time: %2.6f s" % (it, l, time.time() - t_start))
             for n in range(k):
@@ -622,6 +629,8 @@ def hermitian_davidson_gpu(op, nstates, omega_shift=None, ifact=1, conv=1.0e-14,
         w_host = np.sqrt(w2[:k].cpu().numpy())
         stats.history.append((l, nr.copy(), w_host.copy()))
         stats.iterations = it + 1
+        if PROGRESS is not None:
+            PROGRESS("davidson it %d: l=%d max res %.3e (%.1f s)" % (it, l, float(nr.max()), time.time() - t_start))
         if verbose > 0:
             print("Iteration %d: ( %d expansion vectors )   time: %2.6f s" % (it, l, time.time() - t_start))
             for n in range(k):

@@ -50,8 +50,12 @@ def polyene(n, cc_double=1.34, cc_single=1.46, ch=1.09):
     return [(Z, list(np.array(x) * ANGSTROM)) for Z, x in atoms]
 
 
-def graphene_flake(rows=24, cols=25, cc=1.42, ch=1.09):
-    """Rectangular H-terminated graphene flake (rows zigzag chains x cols atoms)."""
+def graphene_flake(rows=24, cols=25, cc=1.42, ch=1.09, prune=False):
+    """Rectangular H-terminated graphene flake (rows zigzag chains x cols atoms).
+
+    prune=True removes carbons left with a single carbon neighbour (the flake
+    corners) before capping, so every carbon is sp2 with one H at most.
+    """
     a = cc
     pos = {}
     for r in range(rows):
@@ -60,11 +64,17 @@ def graphene_flake(rows=24, cols=25, cc=1.42, ch=1.09):
             y = r * 1.5 * a + (0.5 * a if (j + r) % 2 == 1 else 0.0)
             pos[(r, j)] = np.array([x, y])
     keys = list(pos.keys())
-    index = {k: i for i, k in enumerate(keys)}
     cxy = np.array([pos[k] for k in keys])
-    nat = len(keys)
-    d = np.linalg.norm(cxy[:, None, :] - cxy[None, :, :], axis=-1)
-    bonded = (d < 1.1 * a) & (d > 0.1)
+    while True:
+        d = np.linalg.norm(cxy[:, None, :] - cxy[None, :, :], axis=-1)
+        bonded = (d < 1.1 * a) & (d > 0.1)
+        if not prune:
+            break
+        deg = bonded.sum(1)
+        if (deg > 1).all():
+            break
+        cxy = cxy[deg > 1]
+    nat = len(cxy)
     hxy = []
     for i in range(nat):
         nb = np.where(bonded[i])[0]
@@ -79,7 +89,6 @@ def graphene_flake(rows=24, cols=25, cc=1.42, ch=1.09):
     if len(hxy) % 2 == 1:
         # closed shell needs an even electron count (4 per C, 1 per H)
         hxy = hxy[:-1]
-    del index
     atoms = [(6, [p[0], p[1], 0.0]) for p in cxy] + [(1, [p[0], p[1], 0.0]) for p in hxy]
     return [(Z, list(np.array(x) * ANGSTROM)) for Z, x in atoms]
 
