@@ -80,6 +80,16 @@ _SIGS = {
     "dftb_syevx": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _sz, _vp]),
     "dftb_unit_vectors": (_i, [_vp, _i, _l, _vp, _vp]),
     "dftb_symmetrize": (_i, [_vp, _i, _i, _vp, _vp]),
+    "dftb_coldot": (_i, [_vp, _vp, _i, _l, _vp, _vp]),
+    "dftb_pair_features": (_i, [_vp, _vp, _vp, _vp, _vp, _l, _i, _vp, _vp]),
+    "dftb_restricted_ab": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "dftb_mlp_bias_act": (_i, [_vp, _vp, _l, _i, _i, _vp, _vp]),
+    "dftb_mlp_act_bwd": (_i, [_vp, _vp, _l, _vp]),
+    "dftb_mlp_colsum": (_i, [_vp, _l, _i, _vp, _vp]),
+    "dftb_mlp_mse": (_i, [_vp, _vp, _vp, _l, _i, _vp, _vp, _i, _vp]),
+    "dftb_adam": (_i, [_vp, _vp, _vp, _vp, _l, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                       _i, ctypes.c_float, _vp]),
+    "dftb_mlp_normalize": (_i, [_vp, _l, _i, _vp, _vp, _vp]),
 }
 
 EXPORTED = sorted(_SIGS)

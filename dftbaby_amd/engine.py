@@ -404,7 +404,8 @@ def _orthonormalize_block(bs, l, m, N, ws, st):
 
 
 def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, XmYguess=None, XpYguess=None,
-                              w_guess=None, precision=0, verbose=0, refine=True, stats=None):
+                              w_guess=None, precision=0, verbose=0, refine=True, stats=None, initial_space=None,
+                              precond_omega=None):
     """Non-Hermitian Davidson on the GPU (Solver.py:637-917).
 
     Returns padded device tensors (Omega [k], C, XmY, XpY [k][N]) and stats.
@@ -417,7 +418,10 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
     kmax = o * v
     st = _stream()
     ws = op.ws
-    if XpYguess is None:
+    if initial_space is not None:
+        XpYguess = XmYguess = None
+        lmax = int(initial_space.shape[0])
+    elif XpYguess is None:
         lmax = min(ifact * k, kmax)
     else:
         lmax = k + min(k, kmax - k)
@@ -425,7 +429,10 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
     sub = _Subspace(N, cap, dev, 2)
     bs, bp, bm = sub.bs, sub.prod[0], sub.prod[1]
     Mp, Mm = sub.M
-    if XpYguess is None:
+    if initial_space is not None:
+        bs[:lmax] = initial_space
+        lmax = _orthonormalize_block(bs, 0, lmax, N, ws, st)
+    elif XpYguess is None:
         ii, aa = _initial_indices(op.omega_real[:o, :v].cpu().numpy(), o, v, lmax)
         idx = torch.as_tensor(ii * op.vpad + aa, dtype=torch.int64, device=dev)
         call("dftb_unit_vectors", _p(idx), lmax, N, _p(bs), st)
@@ -529,7 +536,8 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
         wsel = torch.cat([wdev, wdev])
         seli = torch.as_tensor(sel, dtype=torch.int32, device=dev)
         Q = bs[l:l + dk]
-        call("dftb_precondition", _p(W), _p(wsel), _p(seli), dk, _p(op.omega), N, _p(Q), st)
+        pom = op.omega if precond_omega is None else precond_omega
+        call("dftb_precondition", _p(W), _p(wsel), _p(seli), dk, _p(pom), N, _p(Q), st)
         nb = _lib.load().dftb_orthonormalize_workspace(l, dk, N)
         wsb, wsn = ws.get(nb)
         call("dftb_orthonormalize", _p(bs), l, _p(Q), dk, N, 1.0e-8, _p(keep), _p(wsb), wsn, st)

@@ -199,6 +199,29 @@ int dftb_oia_normalize(double* O, const double* Oii, const double* Oaa, int nocc
 /* Unit guess vectors at the given flat indices (Solver.py:232-251). */
 int dftb_unit_vectors(const long* idx, int k, long n, double* X, void* stream);
 
+/* ---- learned initial space / preconditioner ---------------------------------
+ * No counterpart in the reference: the engine's learned part (config of the
+ * learned Davidson guess).  out[x] = sum_A P[A][x] Q[A][x] (P, Q: [nat][N]). */
+int dftb_coldot(const double* P, const double* Q, int nat, long N, double* out, void* stream);
+/* Per-pair features [N][nf] (float): omega, ||q||, A_ia,ia, B_ia,ia, A-B, sqrt((A+B)(A-B)). */
+int dftb_pair_features(const double* omega, const double* qn2, const double* cc, const double* cl,
+                       const double* xl, long N, int nf, float* F, void* stream);
+/* (A+-B) restricted to M selected pairs (ip[p], ap[p]) from the engine-layout charges;
+ * Kc = Coulomb block (p|q). */
+int dftb_restricted_ab(const int* ip, const int* ap, int M, int nat, int opad, int vpad, const double* omega,
+                       const double* Kc, const double* q_oo, const double* q_ov, const double* qf_ov,
+                       const double* qf_vv, int lc, double* ApB, double* AmB, void* stream);
+/* MLP steps (FP32): Z += b, H = silu(Z) (act) or Z; G *= silu'(Z); db = column sums of G;
+ * weighted MSE loss partials and its gradient; Adam; feature normalisation. */
+int dftb_mlp_bias_act(float* Z, const float* b, long rows, int cols, int act, float* H, void* stream);
+int dftb_mlp_act_bwd(float* G, const float* Z, long n, void* stream);
+int dftb_mlp_colsum(const float* G, long rows, int cols, float* db, void* stream);
+int dftb_mlp_mse(const float* Y, const float* T, const float* w, long rows, int nout, float* G,
+                 float* partial, int nblocks, void* stream);
+int dftb_adam(float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
+              int step, float wd, void* stream);
+int dftb_mlp_normalize(float* X, long rows, int cols, const float* mean, const float* inv_std, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
