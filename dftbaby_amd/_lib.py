@@ -47,6 +47,15 @@ class CasidaOperator(ctypes.Structure):
                 ("q_oo", _vp * 2), ("q_ov", _vp * 2), ("qf_ov", _vp * 2), ("qf_vv", _vp * 2)]
 
 
+class EnsMolecule(ctypes.Structure):
+    _fields_ = [("nat", _i), ("nocc", _i), ("nvirt", _i), ("opad", _i), ("vpad", _i), ("k", _i), ("cap", _i),
+                ("nguess", _i), ("lc", _i), ("precision", _i),
+                ("omega", _vp), ("gamma", _vp), ("q_c", _vp), ("q_oo", _vp), ("q_ov", _vp), ("qf_ov", _vp),
+                ("qf_vv", _vp), ("q32", _vp * 4), ("guess", _vp),
+                ("bs", _vp), ("bp", _vp), ("bm", _vp), ("Mp", _vp), ("Mm", _vp), ("work", _vp),
+                ("w", _vp), ("C", _vp), ("XmY", _vp), ("XpY", _vp), ("resid", _vp), ("status", _vp)]
+
+
 _SIGS = {
     "dftb_gemm": (_i, [ctypes.POINTER(GemmArgs), _vp]),
     "dftb_trans_charges_workspace": (_sz, [_i, _i, _i, _i]),
@@ -81,6 +90,8 @@ _SIGS = {
     "dftb_unit_vectors": (_i, [_vp, _i, _l, _vp, _vp]),
     "dftb_symmetrize": (_i, [_vp, _i, _i, _vp, _vp]),
     "dftb_coldot": (_i, [_vp, _vp, _i, _l, _vp, _vp]),
+    "dftb_ensemble_workspace": (_sz, [ctypes.POINTER(EnsMolecule)]),
+    "dftb_ensemble_solve": (_i, [_vp, _i, _d, _i, _vp]),
     "dftb_pair_features": (_i, [_vp, _vp, _vp, _vp, _vp, _l, _i, _vp, _vp]),
     "dftb_restricted_ab": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
     "dftb_mlp_bias_act": (_i, [_vp, _vp, _l, _i, _i, _vp, _vp]),

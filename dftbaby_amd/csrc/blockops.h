@@ -1,0 +1,312 @@
+// Block-level (one workgroup) FP64 building blocks shared by the single-molecule
+// kernels and the persistent ensemble kernel.
+#pragma once
+#include <math.h>
+#include "common.h"
+
+#define EIG_SMALL_CLUSTER 8
+
+__host__ __device__ static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct EigLayout {
+  size_t W, d, e, tau, z, scratch, coef, total;
+};
+__host__ __device__ static EigLayout eig_layout(int l, int nev) {
+  EigLayout L;
+  size_t off = 0;
+  L.W = off; off += al256((size_t)l * l * 8);
+  L.d = off; off += al256((size_t)l * 8);
+  L.e = off; off += al256((size_t)l * 8);
+  L.tau = off; off += al256((size_t)l * 8);
+  L.z = off; off += al256((size_t)l * nev * 8);
+  L.scratch = off; off += al256((size_t)6 * l * nev * 8);
+  L.coef = off; off += al256((size_t)nev * 8);
+  L.total = off;
+  return L;
+}
+
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ double blk_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
+  return s;
+}
+
+// Sturm count: number of eigenvalues of the tridiagonal (d, e) below x.
+__device__ int sturm_count(const double* d, const double* e, int l, double x, double pivmin) {
+  int c = 0;
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  if (q < 0) ++c;
+  for (int i = 1; i < l; ++i) {
+    q = d[i] - x - e[i - 1] * e[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0) ++c;
+  }
+  return c;
+}
+
+__device__ __forceinline__ double hash_unit(unsigned int a, unsigned int b) {
+  unsigned int h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u;
+  h ^= h >> 15; h *= 0x2C1B3C6Du; h ^= h >> 12; h *= 0x297A2D39u; h ^= h >> 15;
+  return ((double)(h & 0xFFFFFF) / (double)0xFFFFFF) * 2.0 - 1.0;
+}
+
+// Solve (T - lam I) y = x in place (x -> y) with a partial-pivot LU.
+__device__ void tri_solve(const double* d, const double* e, int l, double lam, double tiny, double* u0, double* u1,
+                          double* u2, double* mlt, int* piv, double* x) {
+  double r0 = d[0] - lam, r1 = (l > 1) ? e[0] : 0.0;
+  for (int k = 0; k < l - 1; ++k) {
+    const double sub = e[k];
+    const double a1 = d[k + 1] - lam;
+    const double b1 = (k + 1 < l - 1) ? e[k + 1] : 0.0;
+    if (fabs(r0) >= fabs(sub)) {
+      if (fabs(r0) < tiny) r0 = (r0 >= 0 ? tiny : -tiny);
+      double m = sub / r0;
+      u0[k] = r0; u1[k] = r1; u2[k] = 0.0; mlt[k] = m; piv[k] = 0;
+      r0 = a1 - m * r1;
+      r1 = b1;
+    } else {
+      double m = r0 / sub;
+      u0[k] = sub; u1[k] = a1; u2[k] = b1; mlt[k] = m; piv[k] = 1;
+      r0 = r1 - m * a1;
+      r1 = -m * b1;
+    }
+  }
+  if (fabs(r0) < tiny) r0 = (r0 >= 0 ? tiny : -tiny);
+  u0[l - 1] = r0;
+  // forward
+  for (int k = 0; k < l - 1; ++k) {
+    if (piv[k]) {
+      double t = x[k];
+      x[k] = x[k + 1];
+      x[k + 1] = t - mlt[k] * x[k + 1];
+    } else {
+      x[k + 1] -= mlt[k] * x[k];
+    }
+  }
+  // backward
+  for (int k = l - 1; k >= 0; --k) {
+    double s = x[k];
+    if (k + 1 < l) s -= u1[k] * x[k + 1];
+    if (k + 2 < l) s -= u2[k] * x[k + 2];
+    x[k] = s / u0[k];
+  }
+}
+
+
+// Lowest nev eigenpairs of the symmetric l x l matrix A (leading dim ld), one
+// workgroup: Householder tridiagonalisation, Sturm bisection, inverse
+// iteration (clusters re-orthogonalised), back-transformation.  sv, sp: LDS
+// scratch of at least l doubles each; sh: >= blockDim/64 doubles; s_bnd: 2.
+__device__ __noinline__ void syevx_block(const double* __restrict__ A, int l, int ld, int nev,
+                                         double* __restrict__ evals, double* __restrict__ evecs,
+                                         char* __restrict__ ws, int* __restrict__ info, double* sv, double* sp,
+                                         double* sh, double* s_bnd) {
+  EigLayout Lo = eig_layout(l, nev);
+  double* W = (double*)(ws + Lo.W);
+  double* d = (double*)(ws + Lo.d);
+  double* e = (double*)(ws + Lo.e);
+  double* tau = (double*)(ws + Lo.tau);
+  double* Z = (double*)(ws + Lo.z);
+  double* scr = (double*)(ws + Lo.scratch);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+
+  for (long i = tid; i < (long)l * l; i += blockDim.x) {
+    long r = i / l, c = i % l;
+    W[i] = 0.5 * (A[r * ld + c] + A[c * ld + r]);
+  }
+  __syncthreads();
+
+  // ---- 1. Householder tridiagonalisation
+  for (int j = 0; j + 2 < l; ++j) {
+    const int n2 = l - j - 1;
+    double part = 0.0;
+    for (int r = tid; r < n2; r += blockDim.x) {
+      double x = W[(long)(j + 1 + r) * l + j];
+      sv[r] = x;
+      part += x * x;
+    }
+    const double alpha2 = blk_sum(part, sh);
+    const double x0 = sv[0];
+    __syncthreads();
+    if (alpha2 <= 1e-300) {
+      if (tid == 0) { tau[j] = 0.0; e[j] = 0.0; d[j] = W[(long)j * l + j]; }
+      for (int r = tid; r < n2; r += blockDim.x) W[(long)(j + 1 + r) * l + j] = 0.0;
+      __syncthreads();
+      continue;
+    }
+    const double alpha = sqrt(alpha2);
+    const double beta = (x0 >= 0.0) ? -alpha : alpha;
+    const double v0 = x0 - beta;
+    const double vtv = alpha2 - x0 * x0 + v0 * v0;
+    const double t = 2.0 / vtv;
+    if (tid == 0) { sv[0] = v0; tau[j] = t; e[j] = beta; d[j] = W[(long)j * l + j]; }
+    __syncthreads();
+    // p = t * A22 v  (one wave per row, lanes across columns)
+    for (int r = wv; r < n2; r += nw) {
+      const double* row = W + (long)(j + 1 + r) * l + (j + 1);
+      double s = 0.0;
+      for (int c = lane; c < n2; c += 64) s += row[c] * sv[c];
+      s = wave_sum(s);
+      if (lane == 0) sp[r] = t * s;
+    }
+    __syncthreads();
+    part = 0.0;
+    for (int r = tid; r < n2; r += blockDim.x) part += sp[r] * sv[r];
+    const double K = 0.5 * t * blk_sum(part, sh);
+    for (int r = tid; r < n2; r += blockDim.x) sp[r] -= K * sv[r];
+    __syncthreads();
+    for (long i = tid; i < (long)n2 * n2; i += blockDim.x) {
+      int r = (int)(i / n2), c = (int)(i % n2);
+      W[(long)(j + 1 + r) * l + (j + 1 + c)] -= sv[r] * sp[c] + sp[r] * sv[c];
+    }
+    for (int r = tid; r < n2; r += blockDim.x) W[(long)(j + 1 + r) * l + j] = sv[r];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (l >= 2) {
+      d[l - 2] = W[(long)(l - 2) * l + (l - 2)];
+      e[l - 2] = W[(long)(l - 1) * l + (l - 2)];
+    }
+    d[l - 1] = W[(long)(l - 1) * l + (l - 1)];
+    if (l >= 1) e[l - 1] = 0.0;
+    // Gershgorin bounds
+    double lo = 1e300, hi = -1e300;
+    for (int i = 0; i < l; ++i) {
+      double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < l - 1 ? fabs(e[i]) : 0.0);
+      lo = fmin(lo, d[i] - r);
+      hi = fmax(hi, d[i] + r);
+    }
+    s_bnd[0] = lo;
+    s_bnd[1] = hi;
+  }
+  __syncthreads();
+  const double lo0 = s_bnd[0], hi0 = s_bnd[1];
+  const double tnorm = fmax(fabs(lo0), fabs(hi0));
+  double emax2 = 0.0;
+  for (int i = 0; i + 1 < l; ++i) emax2 = fmax(emax2, e[i] * e[i]);
+  const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax2);
+  const double eps = 1.1102230246251565e-16;
+
+  // ---- 2. bisection for the nev lowest eigenvalues
+  for (int m = tid; m < nev; m += blockDim.x) {
+    double a = lo0 - 2.0 * eps * tnorm - pivmin, b = hi0 + 2.0 * eps * tnorm + pivmin;
+    for (int it = 0; it < 200; ++it) {
+      double mid = 0.5 * (a + b);
+      if (sturm_count(d, e, l, mid, pivmin) > m) b = mid; else a = mid;
+      if (b - a <= 2.0 * eps * fmax(fabs(a), fabs(b)) + pivmin) break;
+    }
+    evals[m] = 0.5 * (a + b);
+  }
+  __syncthreads();
+
+  // ---- 3. inverse iteration.  Clusters (gap < 1e-3 ||T||) of at most EIG_SMALL_CLUSTER
+  // members are handled by the thread owning their first member; larger clusters
+  // are processed by the whole workgroup with classical Gram-Schmidt (twice)
+  // against the earlier members in every iteration.
+  const double ortol = 1.0e-3 * tnorm;
+  const double tiny = eps * fmax(tnorm, 1e-300);
+  double* coef = (double*)(ws + Lo.coef);
+  for (int m0 = tid; m0 < nev; m0 += blockDim.x) {
+    if (m0 > 0 && evals[m0] - evals[m0 - 1] < ortol) continue;  // not a cluster head
+    int m1 = m0 + 1;
+    while (m1 < nev && evals[m1] - evals[m1 - 1] < ortol) ++m1;
+    if (m1 - m0 > EIG_SMALL_CLUSTER) continue;
+    double* base = scr + (size_t)6 * l * m0;
+    double* u0 = base;
+    double* u1 = base + l;
+    double* u2 = base + 2 * l;
+    double* mlt = base + 3 * l;
+    int* piv = (int*)(base + 4 * l);
+    double* x = base + 5 * l;
+    for (int m = m0; m < m1; ++m) {
+      double lam = evals[m] + (double)(m - m0) * 10.0 * tiny;
+      for (int i = 0; i < l; ++i) x[i] = hash_unit((unsigned)m, (unsigned)i);
+      for (int it = 0; it < 4; ++it) {
+        tri_solve(d, e, l, lam, tiny, u0, u1, u2, mlt, piv, x);
+        for (int mm = m0; mm < m; ++mm) {
+          double s = 0.0;
+          for (int i = 0; i < l; ++i) s += Z[(long)i * nev + mm] * x[i];
+          for (int i = 0; i < l; ++i) x[i] -= s * Z[(long)i * nev + mm];
+        }
+        double nrm = 0.0;
+        for (int i = 0; i < l; ++i) nrm += x[i] * x[i];
+        nrm = sqrt(nrm);
+        if (!(nrm > 0.0)) nrm = 1.0;
+        for (int i = 0; i < l; ++i) x[i] /= nrm;
+      }
+      for (int i = 0; i < l; ++i) Z[(long)i * nev + m] = x[i];
+    }
+  }
+  __syncthreads();
+  // large clusters, cooperatively
+  for (int m0 = 0; m0 < nev;) {
+    int m1 = m0 + 1;
+    while (m1 < nev && evals[m1] - evals[m1 - 1] < ortol) ++m1;
+    if (m1 - m0 > EIG_SMALL_CLUSTER) {
+      double* base = scr + (size_t)6 * l * m0;
+      double* x = base + 5 * l;
+      for (int m = m0; m < m1; ++m) {
+        const double lam = evals[m] + (double)(m - m0) * 10.0 * tiny;
+        for (int i = tid; i < l; i += blockDim.x) x[i] = hash_unit((unsigned)m, (unsigned)i);
+        __syncthreads();
+        for (int it = 0; it < 3; ++it) {
+          if (tid == 0) tri_solve(d, e, l, lam, tiny, base, base + l, base + 2 * l, base + 3 * l, (int*)(base + 4 * l), x);
+          __syncthreads();
+          for (int pass = 0; pass < 2 && m > m0; ++pass) {
+            for (int mm = m0 + tid; mm < m; mm += blockDim.x) {
+              double s = 0.0;
+              for (int i = 0; i < l; ++i) s += Z[(long)i * nev + mm] * x[i];
+              coef[mm] = s;
+            }
+            __syncthreads();
+            for (int i = tid; i < l; i += blockDim.x) {
+              double s = x[i];
+              for (int mm = m0; mm < m; ++mm) s -= coef[mm] * Z[(long)i * nev + mm];
+              x[i] = s;
+            }
+            __syncthreads();
+          }
+          double part = 0.0;
+          for (int i = tid; i < l; i += blockDim.x) part += x[i] * x[i];
+          double nrm = sqrt(blk_sum(part, sh));
+          if (!(nrm > 0.0)) nrm = 1.0;
+          for (int i = tid; i < l; i += blockDim.x) x[i] /= nrm;
+          __syncthreads();
+        }
+        for (int i = tid; i < l; i += blockDim.x) Z[(long)i * nev + m] = x[i];
+        __syncthreads();
+      }
+    }
+    m0 = m1;
+  }
+  __syncthreads();
+
+  // ---- 4. back-transformation z <- H_0 ... H_{l-3} z
+  for (int j = l - 3; j >= 0; --j) {
+    const int n2 = l - j - 1;
+    const double t = tau[j];
+    if (t == 0.0) continue;
+    for (int r = tid; r < n2; r += blockDim.x) sv[r] = W[(long)(j + 1 + r) * l + j];
+    __syncthreads();
+    for (int n = wv; n < nev; n += nw) {
+      double s = 0.0;
+      for (int r = lane; r < n2; r += 64) s += sv[r] * Z[(long)(j + 1 + r) * nev + n];
+      s = t * wave_sum(s);
+      for (int r = lane; r < n2; r += 64) Z[(long)(j + 1 + r) * nev + n] -= s * sv[r];
+    }
+    __syncthreads();
+  }
+  for (long i = tid; i < (long)l * nev; i += blockDim.x) evecs[i] = Z[i];
+}

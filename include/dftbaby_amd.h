@@ -199,6 +199,38 @@ int dftb_oia_normalize(double* O, const double* Oii, const double* Oaa, int nocc
 /* Unit guess vectors at the given flat indices (Solver.py:232-251). */
 int dftb_unit_vectors(const long* idx, int k, long n, double* X, void* stream);
 
+/* ---- batched ensemble: one persistent launch ---------------------------------
+ * Solves the lowest k roots of the non-Hermitian Casida problem (the algorithm
+ * of Solver.py:637-917) for many molecules of uneven size in one launch: one
+ * workgroup per molecule runs its whole Davidson loop on the device.  The host
+ * fills one EnsMolecule per molecule (device array, largest first): charges in
+ * the engine layouts (FP64, plus FP32 copies q32 = {q_oo, q_ov, qf_ov, qf_vv}
+ * for precision 1), nguess initial unit-vector indices, the expansion storage
+ * (cap <= 1024 vectors), a workspace of dftb_ensemble_workspace() bytes and the
+ * outputs.  status = {iterations, code, final subspace size}; code 1 =
+ * converged, 0 = maxiter reached, -3 = (A-B) projection not positive definite,
+ * -4 = Mh not Hermitian, -6 = expansion capacity exhausted. */
+typedef struct EnsMolecule {
+  int nat, nocc, nvirt, opad, vpad, k, cap, nguess, lc, precision;
+  const double* omega;                  /* [opad*vpad], padding 1.0 */
+  const double* gamma;                  /* [nat][nat] */
+  const double* q_c;                    /* [nat][opad][vpad] */
+  const double* q_oo;                   /* [nocc][nat][opad] */
+  const double* q_ov;                   /* [nocc][nat][vpad] */
+  const double* qf_ov;                  /* [nocc][nat][vpad] */
+  const double* qf_vv;                  /* [vpad][nat][vpad] */
+  const void* q32[4];                   /* FP32 copies (precision 1) */
+  const long* guess;                    /* [nguess] flat padded indices */
+  double *bs, *bp, *bm;                 /* [cap][opad*vpad] */
+  double *Mp, *Mm;                      /* [cap][cap] */
+  void* work;                           /* dftb_ensemble_workspace() bytes */
+  double *w, *C, *XmY, *XpY, *resid;    /* outputs: [k], [k][N] x3, [k] */
+  int* status;                          /* [3] */
+} EnsMolecule;
+
+size_t dftb_ensemble_workspace(const EnsMolecule* m);
+int dftb_ensemble_solve(const EnsMolecule* mols_dev, int nmol, double conv, int maxiter, void* stream);
+
 /* ---- learned initial space / preconditioner ---------------------------------
  * No counterpart in the reference: the engine's learned part (config of the
  * learned Davidson guess).  out[x] = sum_A P[A][x] Q[A][x] (P, Q: [nat][N]). */

@@ -1,0 +1,38 @@
+"""Persistent ensemble solver vs the oracle Davidson, molecule by molecule (run with -m gpu)."""
+import numpy as np
+import pytest
+
+from conftest import ground_state
+from oracle import tddftb_oracle as O
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+MEMBERS = [("C8H10", 0.05, 1), ("C12H14", 0.05, 2), ("C8H10", 0.05, 3), ("C20H22", 0.05, 4), ("C12H14", 0.05, 5)]
+
+
+def _oracle(gs, k, conv):
+    r = O.solve_from_ground_state(gs, nstates=k, diag_conv=conv, diag_maxiter=100)
+    return r["Omega"], r["XpY"]
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-9), (1, 1e-6)])
+def test_ensemble_matches_oracle(precision, tol):
+    from dftbaby_amd.ensemble import build_operators, solve_operators
+    gss = [ground_state(n, 1, s, seed) for (n, s, seed) in MEMBERS]
+    ops = build_operators(gss)
+    k = 6
+    res = solve_operators(ops, k, conv=1e-8, maxiter=100, precision=precision)
+    for gs, w, xpy, it in zip(gss, res.Omega, res.XpY, res.iterations):
+        ref_w, ref_xpy = _oracle(gs, k, 1e-8)
+        assert np.abs(w - ref_w).max() < tol, (w, ref_w)
+        for a, b in zip(xpy, ref_xpy):
+            assert min(np.linalg.norm(a - b), np.linalg.norm(a + b)) < (1e-5 if precision == 0 else 1e-3)
+
+
+def test_ensemble_reports_nonconvergence():
+    from dftbaby_amd.ensemble import build_operators, solve_operators
+    from dftbaby_amd.engine import ExcitedStatesNotConverged
+    ops = build_operators([ground_state("C8H10", 1, 0.05, 1)])
+    with pytest.raises(ExcitedStatesNotConverged):
+        solve_operators(ops, 4, conv=1e-14, maxiter=1)

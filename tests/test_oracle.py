@@ -212,3 +212,22 @@ def test_davidson_not_converged_raises():
     gs = ground_state("C8H10", 1, 0.05, 9)
     with pytest.raises(O.ExcitedStatesNotConverged):
         O.solve_from_ground_state(gs, nstates=5, diag_conv=1e-14, diag_maxiter=1)
+
+
+def test_mlp_oracle_gradient_finite_differences():
+    """The CPU MLP reference against central differences (pins the GPU gradient test)."""
+    from oracle import mlp_oracle
+    rng = np.random.default_rng(0)
+    shapes = [(6, 8), (8,), (8, 8), (8,), (8, 8), (8,), (8, 2), (2,)]
+    n = sum(int(np.prod(s)) for s in shapes)
+    p = rng.standard_normal(n) * 0.5
+    X = rng.standard_normal((7, 6))
+    T = rng.standard_normal((7, 2))
+    w = np.array([1.0, 0.3])
+    loss, g = mlp_oracle.loss_and_grad(p, shapes, X, T, w)
+    h = 1e-6
+    for i in rng.choice(n, 25, replace=False):
+        e = np.zeros(n)
+        e[i] = h
+        fd = (mlp_oracle.loss_and_grad(p + e, shapes, X, T, w)[0] - mlp_oracle.loss_and_grad(p - e, shapes, X, T, w)[0]) / (2 * h)
+        assert abs(fd - g[i]) < 1e-6 * max(1.0, abs(fd))
