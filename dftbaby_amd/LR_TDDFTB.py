@@ -79,9 +79,13 @@ class LR_TDDFTB:
         self._Lambda2_diagnostics()
         self.Irreps = ["" for _ in range(len(self.Omega))]
         self._t("properties", t0)
-        if nstates is not None:
-            self.determineActiveSpace()
         if getattr(self.dftb2, "verbose", 0) > 0:
+            # the reference computes these inside getEnergies only to print them
+            # (LR_TDDFTB.py:89-112): done here when output is requested
+            if oszis == "Dipoles" and hasattr(self, "tdipMO_X"):
+                self.getTransitionDipolesExc(len(self.Omega))
+            if nstates is not None:
+                self.determineActiveSpace()
             self._writeExcitations()
         return self.Omega, self.oscillator_strength
 
@@ -322,6 +326,51 @@ class LR_TDDFTB:
         M = torch.empty((3, op.N), dtype=torch.float64, device=dev)
         gemm(pos, op.q["c"], M, 3, op.N, op.nat, 3, op.N, op.N, a_trans=1, tile=64)
         self._tdip_from(M)
+
+    def getTransitionDipolesExc(self, Nst):
+        """Dipole matrix elements between the ground and the lowest Nst-1 excited states
+        (LR_TDDFTB.py:593-697), in e*bohr with electrons counted positive:
+
+          tdip[0,0] = d0 (electronic + nuclear), tdip[0,I] = sum_ia C^I_ia <i|r|a>,
+          tdip[A,B] = <C_B, d0 C_A - D_oo^T C_A + C_A D_vv>   (A, B >= 1),
+
+        the last line being the reference's P0/Pij/Pab contraction rewritten as two
+        GEMMs per component.
+        """
+        op = self.op
+        dev = op.device
+        o, v = op.nocc, op.nvirt
+        occ = np.asarray(self.occupied_orbs)
+        ao, av = np.asarray(self.active_occupied_orbs), np.asarray(self.active_virtual_orbs)
+        tMO = (self.tdipMO_X, self.tdipMO_Y, self.tdipMO_Z)
+        d0_elec = np.array([2.0 * np.trace(t[np.ix_(occ, occ)]) for t in tMO])
+        valence = {1: 1, 6: 4, 7: 5, 8: 6}
+        d0_nuc = -sum(valence[Z] * np.asarray(pos) for Z, pos in self.dftb2.getGeometry())
+        d0 = d0_elec + d0_nuc
+        nex = min(Nst - 1, self._C_dev.shape[0])
+        tdip = np.zeros((Nst, Nst, 3))
+        tdip[0, 0] = d0
+        for c in range(3):
+            tdip[0, 1:nex + 1, c] = (self.tdipX, self.tdipY, self.tdipZ)[c][:nex]
+            tdip[1:nex + 1, 0, c] = tdip[0, 1:nex + 1, c]
+        if nex == 0:
+            return tdip
+        C = self._C_dev[:nex].reshape(nex, op.opad, op.vpad)
+        for c in range(3):
+            Doo = torch.as_tensor(np.ascontiguousarray(tMO[c][np.ix_(ao, ao)]), device=dev)
+            Dvv = torch.as_tensor(np.ascontiguousarray(tMO[c][np.ix_(av, av)]), device=dev)
+            Y = torch.zeros((nex, op.opad, op.vpad), dtype=torch.float64, device=dev)
+            for a in range(nex):
+                Ca = C[a]
+                # Y_a = d0 C_a - D_oo^T C_a + C_a D_vv
+                gemm(Doo, Ca, Y[a], o, v, o, o, op.vpad, op.vpad, a_trans=1, alpha=-1.0)
+                gemm(Ca, Dvv, Y[a], o, v, v, op.vpad, v, op.vpad, beta=1.0)
+                Y[a, :o, :v] += d0[c] * Ca[:o, :v]
+            G = torch.empty((nex, nex), dtype=torch.float64, device=dev)
+            gemm(self._C_dev[:nex], Y.reshape(nex, -1), G, nex, nex, op.N, op.N, op.N, nex, b_trans=1, tile=64)
+            tdip[1:nex + 1, 1:nex + 1, c] = G.cpu().numpy().T
+        self.tdip_exc = tdip
+        return tdip
 
     def getOscillatorStrengths(self):
         """LR_TDDFTB.py:698-705."""

@@ -42,7 +42,8 @@ class GemmArgs(ctypes.Structure):
 
 class CasidaOperator(ctypes.Structure):
     _fields_ = [("nat", _i), ("nocc", _i), ("nvirt", _i), ("opad", _i), ("vpad", _i),
-                ("lc", _i), ("precision", _i), ("atom_chunk", _i),
+                ("lc", _i), ("precision", _i), ("atom_chunk", _i), ("tile_expand", _i), ("tile_contract", _i),
+                ("atom_limit", _i),
                 ("omega", _vp), ("gamma", _vp), ("q_c", _vp),
                 ("q_oo", _vp * 2), ("q_ov", _vp * 2), ("qf_ov", _vp * 2), ("qf_vv", _vp * 2)]
 

@@ -280,8 +280,9 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
     void* T = tu;
     void* Tlo = prec == 2 ? (void*)(tu + tu_elems * 2) : nullptr;
     const long esz = prec == 0 ? 8 : (prec == 1 ? 4 : 2);
-    for (int A0 = 0; prec < 2 && A0 < op->nat; A0 += L.ac) {
-      const int ac = (op->nat - A0 < L.ac) ? (op->nat - A0) : L.ac;
+    const int nat_eff = (op->atom_limit > 0 && op->atom_limit < op->nat) ? op->atom_limit : op->nat;
+    for (int A0 = 0; prec < 2 && A0 < nat_eff; A0 += L.ac) {
+      const int ac = (nat_eff - A0 < L.ac) ? (nat_eff - A0) : L.ac;
       // ---- K1 expand: T[n][a'][j][a] = sum_b V[n][j][b] qf_vv[b][A0+a'][a]
       GemmArgs e = gemm_args_default();
       e.in_type = it; e.out_type = ot;
@@ -290,7 +291,7 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
       e.B2 = prec == 2 ? (const char*)op->qf_vv[1] + A0 * vpad * esz : nullptr;
       e.ldb = nat * vpad; e.sB = 0;
       e.C = T; e.C2 = Tlo; e.ldc = vpad; e.cdiv = vpad; e.cstr = opad * vpad; e.sC = (long)ac * opad * vpad;
-      e.M = (int)opad; e.N = (int)(ac * vpad); e.K = (int)vpad; e.batch = nvec;
+      e.M = (int)opad; e.N = (int)(ac * vpad); e.K = (int)vpad; e.batch = nvec; e.tile = op->tile_expand;
       if ((rc = gemm_launch(e, st))) return rc;
       // ---- K1 contract: X1[n][i][a] += sum_{a',j} q_oo[i][A0+a'][j] T[n][a'][j][a]
       GemmArgs c = gemm_args_default();
@@ -300,7 +301,7 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
       c.lda = nat * opad; c.sA = 0;
       c.B = T; c.B2 = Tlo; c.ldb = vpad; c.sB = (long)ac * opad * vpad;
       c.C = X1; c.ldc = vpad; c.sC = L.N; c.beta = 1.0;
-      c.M = op->nocc; c.N = (int)vpad; c.K = (int)(ac * opad); c.batch = nvec;
+      c.M = op->nocc; c.N = (int)vpad; c.K = (int)(ac * opad); c.batch = nvec; c.tile = op->tile_contract;
       if ((rc = gemm_launch(c, st))) return rc;
       // ---- K2 expand: U[n][a'][b][a] = sum_j V[n][j][b] qf_ov[j][A0+a'][a]
       GemmArgs e2 = gemm_args_default();
@@ -310,7 +311,7 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
       e2.B2 = prec == 2 ? (const char*)op->qf_ov[1] + A0 * vpad * esz : nullptr;
       e2.ldb = nat * vpad; e2.sB = 0;
       e2.C = T; e2.C2 = Tlo; e2.ldc = vpad; e2.cdiv = vpad; e2.cstr = vpad * vpad; e2.sC = (long)ac * vpad * vpad;
-      e2.M = (int)vpad; e2.N = (int)(ac * vpad); e2.K = op->nocc; e2.batch = nvec;
+      e2.M = (int)vpad; e2.N = (int)(ac * vpad); e2.K = op->nocc; e2.batch = nvec; e2.tile = op->tile_expand;
       if ((rc = gemm_launch(e2, st))) return rc;
       // ---- K2 contract: X2[n][i][a] += sum_{a',b} q_ov[i][A0+a'][b] U[n][a'][b][a]
       GemmArgs c2 = gemm_args_default();
@@ -320,7 +321,7 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
       c2.lda = nat * vpad; c2.sA = 0;
       c2.B = T; c2.B2 = Tlo; c2.ldb = vpad; c2.sB = (long)ac * vpad * vpad;
       c2.C = X2; c2.ldc = vpad; c2.sC = L.N; c2.beta = 1.0;
-      c2.M = op->nocc; c2.N = (int)vpad; c2.K = (int)(ac * vpad); c2.batch = nvec;
+      c2.M = op->nocc; c2.N = (int)vpad; c2.K = (int)(ac * vpad); c2.batch = nvec; c2.tile = op->tile_contract;
       if ((rc = gemm_launch(c2, st))) return rc;
     }
     // BF16x3: NT GEMMs on K-contiguous layouts.  qf_vv is [nat][vpad][vpad] and
@@ -328,8 +329,8 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
     // the intermediates are stored transposed, T[n][a][a'][j] and U[n][a][a'][b].
     const unsigned short* VTh = (const unsigned short*)Vlo + total;
     const unsigned short* VTl = VTh + total;
-    for (int A0 = 0; prec == 2 && A0 < op->nat; A0 += L.ac) {
-      const int ac = (op->nat - A0 < L.ac) ? (op->nat - A0) : L.ac;
+    for (int A0 = 0; prec == 2 && A0 < nat_eff; A0 += L.ac) {
+      const int ac = (nat_eff - A0 < L.ac) ? (nat_eff - A0) : L.ac;
       GemmArgs e = gemm_args_default();
       e.in_type = GEMM_IN_BF16X3; e.out_type = GEMM_OUT_BF16PAIR;
       e.A = Vx; e.A2 = Vlo; e.lda = vpad; e.sA = L.N;
@@ -338,7 +339,7 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
       e.b_trans = 1; e.ldb = vpad; e.sB = 0;
       e.C = T; e.C2 = Tlo; e.cdiv = vpad; e.cstr = opad; e.cs2 = (long)ac * opad; e.ldc = 1;
       e.sC = (long)ac * opad * vpad;
-      e.M = (int)opad; e.N = (int)(ac * vpad); e.K = (int)vpad; e.batch = nvec;
+      e.M = (int)opad; e.N = (int)(ac * vpad); e.K = (int)vpad; e.batch = nvec; e.tile = op->tile_expand;
       if ((rc = gemm_launch(e, st))) return rc;
       GemmArgs c = gemm_args_default();
       c.in_type = GEMM_IN_BF16X3; c.out_type = GEMM_OUT_F64;
@@ -347,7 +348,7 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
       c.lda = nat * opad; c.sA = 0;
       c.B = T; c.B2 = Tlo; c.b_trans = 1; c.ldb = (long)ac * opad; c.sB = (long)ac * opad * vpad;
       c.C = X1; c.ldc = vpad; c.sC = L.N; c.beta = 1.0;
-      c.M = op->nocc; c.N = (int)vpad; c.K = (int)(ac * opad); c.batch = nvec;
+      c.M = op->nocc; c.N = (int)vpad; c.K = (int)(ac * opad); c.batch = nvec; c.tile = op->tile_contract;
       if ((rc = gemm_launch(c, st))) return rc;
       GemmArgs e2 = gemm_args_default();
       e2.in_type = GEMM_IN_BF16X3; e2.out_type = GEMM_OUT_BF16PAIR;
@@ -357,7 +358,7 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
       e2.b_trans = 1; e2.ldb = opad; e2.sB = 0;
       e2.C = T; e2.C2 = Tlo; e2.cdiv = vpad; e2.cstr = vpad; e2.cs2 = (long)ac * vpad; e2.ldc = 1;
       e2.sC = (long)ac * vpad * vpad;
-      e2.M = (int)vpad; e2.N = (int)(ac * vpad); e2.K = (int)opad; e2.batch = nvec;
+      e2.M = (int)vpad; e2.N = (int)(ac * vpad); e2.K = (int)opad; e2.batch = nvec; e2.tile = op->tile_expand;
       if ((rc = gemm_launch(e2, st))) return rc;
       GemmArgs c2 = gemm_args_default();
       c2.in_type = GEMM_IN_BF16X3; c2.out_type = GEMM_OUT_F64;
@@ -366,7 +367,7 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
       c2.lda = nat * vpad; c2.sA = 0;
       c2.B = T; c2.B2 = Tlo; c2.b_trans = 1; c2.ldb = (long)ac * vpad; c2.sB = (long)ac * vpad * vpad;
       c2.C = X2; c2.ldc = vpad; c2.sC = L.N; c2.beta = 1.0;
-      c2.M = op->nocc; c2.N = (int)vpad; c2.K = (int)(ac * vpad); c2.batch = nvec;
+      c2.M = op->nocc; c2.N = (int)vpad; c2.K = (int)(ac * vpad); c2.batch = nvec; c2.tile = op->tile_contract;
       if ((rc = gemm_launch(c2, st))) return rc;
     }
     hipLaunchKernelGGL(combine_kernel, sgrid(total), dim3(256), 0, st, op->omega, V, L.N, total, ApBV, AmBV, X1, X2, 1);

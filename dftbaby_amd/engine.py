@@ -144,6 +144,7 @@ class CasidaOperatorGPU:
         self.q = {}          # FP64 engine-layout charges
         self.qx = {}         # low-precision operand copies keyed by precision
         self.qii = self.qaa = None
+        self.tuning = {}     # precision -> {"atom_chunk", "tile_expand", "tile_contract"} (autotune.py)
 
     # -- construction ---------------------------------------------------------
     @classmethod
@@ -263,10 +264,14 @@ class CasidaOperatorGPU:
                 cp["qf_vv"] = (hi, lo)
         self.qx[precision] = cp
 
-    def _struct(self, precision, atom_chunk=0):
+    def _struct(self, precision, atom_chunk=0, tiles=None, atom_limit=0):
         s = _lib.CasidaOperator()
         s.nat, s.nocc, s.nvirt, s.opad, s.vpad = self.nat, self.nocc, self.nvirt, self.opad, self.vpad
-        s.lc, s.precision, s.atom_chunk = self.lc, precision, atom_chunk
+        tuned = self.tuning.get(precision, {})
+        s.lc, s.precision = self.lc, precision
+        s.atom_chunk = atom_chunk or tuned.get("atom_chunk", 0)
+        te, tc = tiles if tiles is not None else (tuned.get("tile_expand", 0), tuned.get("tile_contract", 0))
+        s.tile_expand, s.tile_contract, s.atom_limit = te, tc, atom_limit
         s.omega = self.omega.data_ptr()
         s.gamma = self.gamma.data_ptr()
         if precision == 0:
@@ -286,7 +291,7 @@ class CasidaOperatorGPU:
         return s
 
     # -- products ---------------------------------------------------------------
-    def matvec(self, V, precision=0, out_p=None, out_m=None, atom_chunk=0):
+    def matvec(self, V, precision=0, out_p=None, out_m=None, atom_chunk=0, tiles=None, atom_limit=0):
         """((A+B)V, (A-B)V) for V [m][opad*vpad] (FP64, zero padding)."""
         m = V.shape[0]
         if out_p is None:
@@ -295,7 +300,7 @@ class CasidaOperatorGPU:
             out_m = torch.empty((m, self.N), dtype=torch.float64, device=self.device)
         if m == 0:
             return out_p, out_m
-        s = self._struct(precision, atom_chunk)
+        s = self._struct(precision, atom_chunk, tiles, atom_limit)
         nb = _lib.load().dftb_casida_matvec_workspace(ctypes.byref(s), m)
         w, wn = self.ws.get(nb)
         call("dftb_casida_matvec", ctypes.byref(s), _p(V), m, _p(out_p), _p(out_m), _p(w), wn, _stream())

@@ -100,6 +100,8 @@ typedef struct CasidaOperator {
   int lc;          /* 1: long-range exchange terms on */
   int precision;   /* 0 fp64, 1 fp32, 2 bf16x3 */
   int atom_chunk;  /* atoms per exchange slice (0 = auto) */
+  int tile_expand, tile_contract; /* GEMM tile edge for the expand / contract launches (0 auto, 64, 128) */
+  int atom_limit;  /* > 0: exchange over the first atom_limit atoms only (autotuner timing probe) */
   const double* omega;  /* [opad*vpad], padding 1.0 */
   const double* gamma;  /* [nat*nat] */
   const void* q_c;      /* [nat][opad][vpad]  f64 (precision 0) or f32 */
