@@ -37,3 +37,20 @@ __device__ __forceinline__ unsigned short bf16_bits_rne(float x) {
 __device__ __forceinline__ float bf16_bits_to_float(unsigned short b) {
   return __uint_as_float(((unsigned int)b) << 16);
 }
+
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8 XCDs
+// of the MI355X, each with its own L2.  Map the linear dispatch id to a tile id
+// so that every XCD walks a contiguous chunk of the (x fastest, then y, then z)
+// tile space and neighbouring tiles, which share operand rows, share an L2.
+// The map is a bijection of [0, nx*ny*nz).
+__device__ __forceinline__ void xcd_tile(unsigned& tx, unsigned& ty, unsigned& tz) {
+  const unsigned nx = gridDim.x, ny = gridDim.y, nz = gridDim.z;
+  const unsigned total = nx * ny * nz;
+  const unsigned bid = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+  const unsigned q = total / 8u, r = total % 8u;
+  const unsigned xcd = bid % 8u, slot = bid / 8u;
+  const unsigned t = (xcd < r) ? xcd * (q + 1u) + slot : r * (q + 1u) + (xcd - r) * q + slot;
+  tx = t % nx;
+  ty = (t / nx) % ny;
+  tz = t / (nx * ny);
+}

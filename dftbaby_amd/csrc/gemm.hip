@@ -67,9 +67,11 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs p) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const long n0 = (long)blockIdx.x * BN, m0 = (long)blockIdx.y * BM;
-  const int ks = blockIdx.z % p.ksplit;
-  const long b = blockIdx.z / p.ksplit;
+  unsigned tx, ty, tz;
+  xcd_tile(tx, ty, tz);
+  const long n0 = (long)tx * BN, m0 = (long)ty * BM;
+  const int ks = tz % p.ksplit;
+  const long b = tz / p.ksplit;
   const T* A = (const T*)p.A + b * p.sA;
   const T* B = (const T*)p.B + b * p.sB;
   const long kbeg = (long)ks * p.kchunk;

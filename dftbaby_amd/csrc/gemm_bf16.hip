@@ -52,9 +52,11 @@ __global__ __launch_bounds__(256) void gemm_bf3_kernel(GemmArgs p) {
   __shared__ __attribute__((aligned(16))) u16 Bl[2][BN][BF_LDS_ROW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const long n0 = (long)blockIdx.x * BN, m0 = (long)blockIdx.y * BM;
-  const int ks = blockIdx.z % p.ksplit;
-  const long b = blockIdx.z / p.ksplit;
+  unsigned tx, ty, tz;
+  xcd_tile(tx, ty, tz);
+  const long n0 = (long)tx * BN, m0 = (long)ty * BM;
+  const int ks = tz % p.ksplit;
+  const long b = tz / p.ksplit;
   const u16* Ahi = (const u16*)p.A + b * p.sA;
   const u16* Alo = (const u16*)p.A2 + b * p.sA;
   const u16* Bhi = (const u16*)p.B + b * p.sB;

@@ -368,6 +368,7 @@ class DavidsonStats:
         self.matvec_vectors = 0
         self.matvec_time = 0.0
         self.t_total = 0.0
+        self.learned = False
 
 
 def _precondition_select(norms_L, norms_R, k, l, kmax, conv):
