@@ -366,9 +366,24 @@ class DavidsonStats:
         self.iterations = 0
         self.history = []
         self.matvec_vectors = 0
-        self.matvec_time = 0.0
         self.t_total = 0.0
         self.learned = False
+        self._events = []
+
+    def event_pair(self):
+        """Start/stop events on the launch stream around one block matvec; the
+        start is recorded now, the caller records the stop."""
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        self._events.append((e0, e1))
+        return e0, e1
+
+    @property
+    def matvec_time(self):
+        """Device time (s) spent in the block matvecs."""
+        if self._events:
+            self._events[-1][1].synchronize()
+        return sum(a.elapsed_time(b) for a, b in self._events) * 1e-3
 
 
 def _precondition_select(norms_L, norms_R, k, l, kmax, conv):
@@ -410,7 +425,7 @@ def _orthonormalize_block(bs, l, m, N, ws, st):
 
 
 def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, XmYguess=None, XpYguess=None,
-                              w_guess=None, precision=0, verbose=0, refine=True, stats=None, initial_space=None,
+                              w_guess=None, precision=0, verbose=0, stats=None, initial_space=None,
                               precond_omega=None):
     """Non-Hermitian Davidson on the GPU (Solver.py:637-917).
 
@@ -470,17 +485,15 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
         if XpYguess is None or it > 0:
             m = l - new0
             if m > 0:
-                t0 = time.time()
+                ev = stats.event_pair()
                 op.matvec(bs[new0:l], precision, out_p=bp[new0:l], out_m=bm[new0:l])
+                ev[1].record()
                 stats.matvec_vectors += m
                 # new rows and columns of M+- = bs^T (A+-B) bs
                 for P, M in ((bp, Mp), (bm, Mm)):
                     gram(bs, l, P[new0:l], m, N, M, ws, ldo=cap, o_off=new0)          # M[:l, new]
                     if new0 > 0:
                         gram(bs[new0:l], m, P, new0, N, M, ws, ldo=cap, o_off=new0 * cap)  # M[new, :new0]
-                if verbose > 1:
-                    torch.cuda.synchronize()
-                stats.matvec_time += time.time() - t0
             if precision > 0:
                 # low-precision products: use the symmetric part of M+-, sanity-check the asymmetry
                 for Mx in (Mp, Mm):
@@ -552,7 +565,8 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
         if len(kept) < dk:
             if len(kept) > 0:
                 Q[:len(kept)] = Q[torch.as_tensor(kept, device=dev)].clone()
-        new0 = l
+        # after a warm-started iteration 0 (no products yet) every vector needs its products
+        new0 = l if (XpYguess is None or it > 0) else 0
         l = l + len(kept)
     else:
         raise ExcitedStatesNotConverged(
@@ -609,13 +623,13 @@ def hermitian_davidson_gpu(op, nstates, omega_shift=None, ifact=1, conv=1.0e-14,
     for it in range(maxiter):
         m = l - new0
         if m > 0:
-            t0 = time.time()
+            ev = stats.event_pair()
             op.hermitian_matvec(bs[new0:l], sq, om, precision, out=rbs[new0:l])
+            ev[1].record()
             stats.matvec_vectors += m
             gram(bs, l, rbs[new0:l], m, N, H, ws, ldo=cap, o_off=new0)
             if new0 > 0:
                 gram(bs[new0:l], m, rbs, new0, N, H, ws, ldo=cap, o_off=new0 * cap)
-            stats.matvec_time += time.time() - t0
         nev = l if use_l2 else k
         w2 = torch.empty(nev, dtype=torch.float64, device=dev)
         Tb = torch.empty((l, nev), dtype=torch.float64, device=dev)

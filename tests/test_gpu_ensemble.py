@@ -16,13 +16,15 @@ def _oracle(gs, k, conv):
     return r["Omega"], r["XpY"]
 
 
-@pytest.mark.parametrize("precision,tol", [(0, 1e-9), (1, 1e-6)])
-def test_ensemble_matches_oracle(precision, tol):
+@pytest.mark.parametrize("precision,conv,tol", [(0, 1e-8, 1e-9), (1, 1e-6, 1e-6)])
+def test_ensemble_matches_oracle(precision, conv, tol):
+    """FP64 to 1e-8 residual; FP32 exchange to the ensemble tolerance 1e-6 (the
+    FP32 products put a floor near 1e-7 under the residual), energies within 1e-6 Ha."""
     from dftbaby_amd.ensemble import build_operators, solve_operators
     gss = [ground_state(n, 1, s, seed) for (n, s, seed) in MEMBERS]
     ops = build_operators(gss)
     k = 6
-    res = solve_operators(ops, k, conv=1e-8, maxiter=100, precision=precision)
+    res = solve_operators(ops, k, conv=conv, maxiter=100, precision=precision)
     for gs, w, xpy, it in zip(gss, res.Omega, res.XpY, res.iterations):
         ref_w, ref_xpy = _oracle(gs, k, 1e-8)
         assert np.abs(w - ref_w).max() < tol, (w, ref_w)
