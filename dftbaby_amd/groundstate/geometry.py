@@ -9,6 +9,8 @@ Geometries are lists of ``(Z, [x, y, z])`` in bohr, the atom-list convention of
   chains with ``cols`` carbons each (C-C 1.42 Angstrom); every carbon with
   fewer than three carbon neighbours is capped with hydrogens (C-H 1.09) along
   the missing bond directions.
+* ``graphene_flake_armchair(widths)``: elongated-hexagon flake with armchair
+  edges only (C604H68 by default), H-terminated.
 * ``displace(atoms, sigma_angstrom, seed)``: seeded Gaussian displacement of
   every Cartesian component.
 """
@@ -89,6 +91,62 @@ def graphene_flake(rows=24, cols=25, cc=1.42, ch=1.09, prune=False):
     if len(hxy) % 2 == 1:
         # closed shell needs an even electron count (4 per C, 1 per H)
         hxy = hxy[:-1]
+    atoms = [(6, [p[0], p[1], 0.0]) for p in cxy] + [(1, [p[0], p[1], 0.0]) for p in hxy]
+    return [(Z, list(np.array(x) * ANGSTROM)) for Z, x in atoms]
+
+
+def _cap_hydrogens(cxy, a, ch):
+    """Bond graph of the carbons and one H per missing bond (two for a carbon
+    with a single carbon neighbour), along the bisector of the missing bonds."""
+    d = np.linalg.norm(cxy[:, None, :] - cxy[None, :, :], axis=-1)
+    bonded = (d < 1.1 * a) & (d > 0.1)
+    hxy = []
+    for i in range(len(cxy)):
+        nb = np.where(bonded[i])[0]
+        units = [(cxy[j] - cxy[i]) / np.linalg.norm(cxy[j] - cxy[i]) for j in nb]
+        if len(units) == 2:
+            dvec = -(units[0] + units[1])
+            dvec /= np.linalg.norm(dvec)
+            hxy.append(cxy[i] + ch * dvec)
+        elif len(units) == 1:
+            for ang in (120.0, -120.0):
+                hxy.append(cxy[i] + ch * _rot2(units[0], ang))
+    return bonded, hxy
+
+
+def graphene_flake_armchair(widths=(18.5, 22.5, 22.5), cc=1.42, ch=1.09):
+    """H-terminated graphene flake with armchair edges only.
+
+    The honeycomb (C-C ``cc`` Angstrom) is cut by three pairs of parallel lines
+    along the armchair directions, at distances ``widths`` (Angstrom) from a ring
+    centre, i.e. an elongated hexagon; carbons left with fewer than two carbon
+    neighbours are removed and every two-coordinated carbon gets one H.  The
+    default (18.5, 22.5, 22.5) gives C604H68 (Nat 672, 1242 doubly occupied
+    valence orbitals).  Armchair edges carry no edge states, so the long-range
+    corrected ground state stays stable (A - B positive definite), which
+    zigzag-edged rectangles of this size do not.
+    """
+    a = cc
+    a1 = np.array([np.sqrt(3.0) * a, 0.0])
+    a2 = np.array([np.sqrt(3.0) / 2.0 * a, 1.5 * a])
+    R = int(2 * max(widths) / a) + 4
+    ij = np.array([(i, j) for i in range(-R, R + 1) for j in range(-R, R + 1)], dtype=float)
+    cells = ij[:, :1] * a1 + ij[:, 1:] * a2
+    pts = np.concatenate([cells, cells + np.array([0.0, a])]) - np.array([np.sqrt(3.0) / 2.0 * a, 0.5 * a])
+    keep = np.ones(len(pts), bool)
+    for k, w in enumerate(widths):
+        t = np.deg2rad(60.0 * k)
+        keep &= np.abs(pts @ np.array([np.cos(t), np.sin(t)])) <= w
+    cxy = pts[keep]
+    while True:
+        d = np.linalg.norm(cxy[:, None, :] - cxy[None, :, :], axis=-1)
+        deg = ((d < 1.1 * a) & (d > 0.1)).sum(1)
+        if (deg > 1).all():
+            break
+        cxy = cxy[deg > 1]
+    order = np.lexsort((cxy[:, 0], np.round(cxy[:, 1], 6)))
+    cxy = cxy[order]
+    _, hxy = _cap_hydrogens(cxy, a, ch)
     atoms = [(6, [p[0], p[1], 0.0]) for p in cxy] + [(1, [p[0], p[1], 0.0]) for p in hxy]
     return [(Z, list(np.array(x) * ANGSTROM)) for Z, x in atoms]
 
