@@ -163,6 +163,199 @@ __global__ __launch_bounds__(256) void gemm_bf3_kernel(GemmArgs p) {
       }
 }
 
+// ---------------------------------------------------------------------------
+// Large-tile pipelined variant: 256 x 128 output tile per workgroup (four
+// wave64s, each 128 x 64 = 8 x 4 MFMA tiles), BK = 32, three-stage LDS ring
+// filled by direct buffer->LDS loads (buffer_load_dwordx4 ... lds).  Each wave
+// instruction deposits 64 x 16 bytes contiguously in LDS; the lanes choose
+// their global 16-byte chunk so that LDS row r holds k-chunk c in slot
+// c ^ ((r >> 2) & 3), which makes the 16-byte fragment reads of 16 rows at one
+// k-chunk hit disjoint banks.  Out-of-range rows and k-chunks get an offset
+// past the buffer's num_records and load zeros.  The ring stages are separate
+// LDS arrays so the compiler's own waits for the DMA stay precise; the
+// cross-wave hand-off is an explicit vmcnt wait followed by s_barrier.
+#define P_BM 256
+#define P_BN 128
+#define P_BK 32
+#define P_LOADS 12  // buffer->LDS instructions per thread per stage (A hi/lo 4+4, B hi/lo 2+2)
+
+typedef int i4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) u16 lds_u16_t;
+
+// raw buffer descriptor (stride 0, 32-bit data format) over [base, base + bytes)
+__device__ __forceinline__ i4_t make_rsrc(const void* base, unsigned bytes) {
+  const unsigned long long a = (unsigned long long)base;
+  i4_t r;
+  r.x = (int)(a & 0xffffffffu);
+  r.y = (int)((a >> 32) & 0xffffu);
+  r.z = (int)bytes;
+  r.w = 0x00020000;
+  return r;
+}
+
+__device__ __forceinline__ unsigned lds_addr(const u16* p) { return (unsigned)(unsigned long long)(lds_u16_t*)p; }
+
+// 16 bytes per lane from buffer offset `off` into LDS at m0 + 16 * lane.  Issued
+// as inline asm so that the compiler's waitcnt insertion does not serialise the
+// prefetch behind later LDS reads; completion is ordered by P_WAIT_BARRIER.
+__device__ __forceinline__ void dma16(i4_t rs, unsigned lds, unsigned off) {
+  asm volatile("s_mov_b32 m0, %0\n\tbuffer_load_dwordx4 %1, %2, 0 offen lds" ::"s"(lds), "v"(off), "s"(rs)
+               : "memory");
+}
+
+// wait until at most N buffer loads of this wave are outstanding, then barrier
+#define P_WAIT_BARRIER(N) asm volatile("s_waitcnt vmcnt(" #N ")\n\ts_barrier" ::: "memory")
+
+__global__ __launch_bounds__(256) void gemm_bf3_pipe_kernel(GemmArgs p) {
+  __shared__ __attribute__((aligned(1024))) u16 a0h[P_BM * P_BK], a0l[P_BM * P_BK], b0h[P_BN * P_BK], b0l[P_BN * P_BK];
+  __shared__ __attribute__((aligned(1024))) u16 a1h[P_BM * P_BK], a1l[P_BM * P_BK], b1h[P_BN * P_BK], b1l[P_BN * P_BK];
+  __shared__ __attribute__((aligned(1024))) u16 a2h[P_BM * P_BK], a2l[P_BM * P_BK], b2h[P_BN * P_BK], b2l[P_BN * P_BK];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: the LDS destination goes to M0
+  const int wm = wave >> 1, wn = wave & 1;
+  unsigned tx, ty, tz;
+  xcd_tile(tx, ty, tz);
+  const long n0 = (long)tx * P_BN, m0 = (long)ty * P_BM;
+  const int ks = tz % p.ksplit;
+  const long b = tz / p.ksplit;
+  const long kbeg = (long)ks * p.kchunk;
+  const long kend = min((long)p.K, kbeg + p.kchunk);
+  const long M = p.M, N = p.N;
+  const long mrows = min((long)P_BM, M - m0), nrows = min((long)P_BN, N - n0);
+
+  // buffer descriptors over this workgroup's rows; an offset equal to num_records loads zeros
+  const unsigned nbA = (unsigned)(((mrows - 1) * p.lda + kend) * 2);
+  const unsigned nbB = (unsigned)(((nrows - 1) * p.ldb + kend) * 2);
+  const i4_t rah = make_rsrc((const u16*)p.A + b * p.sA + m0 * p.lda, nbA);
+  const i4_t ral = make_rsrc((const u16*)p.A2 + b * p.sA + m0 * p.lda, nbA);
+  const i4_t rbh = make_rsrc((const u16*)p.B + b * p.sB + n0 * p.ldb, nbB);
+  const i4_t rbl = make_rsrc((const u16*)p.B2 + b * p.sB + n0 * p.ldb, nbB);
+  // this lane's chunk: row 16 * ch + lane / 4 of the tile, k-chunk (lane & 3) ^ ((lane >> 4) & 3)
+  const int cl = (lane & 3) ^ ((lane >> 4) & 3);
+  unsigned oA0, oA1, oA2, oA3, oB0, oB1;
+  {
+    const long r = lane >> 2;
+    auto offa = [&](int i) -> unsigned {
+      const long rr = (wave * 4 + i) * 16 + r;
+      return rr < mrows ? (unsigned)((rr * p.lda + kbeg + 8 * cl) * 2) : nbA;
+    };
+    auto offb = [&](int i) -> unsigned {
+      const long rr = (wave * 2 + i) * 16 + r;
+      return rr < nrows ? (unsigned)((rr * p.ldb + kbeg + 8 * cl) * 2) : nbB;
+    };
+    oA0 = offa(0); oA1 = offa(1); oA2 = offa(2); oA3 = offa(3);
+    oB0 = offb(0); oB1 = offb(1);
+  }
+  const long kl = kbeg + 8 * cl;  // k of this lane's chunk at stage 0
+
+#define P_LD(rs, base, ch, off) dma16(rs, lds_addr((base) + (ch) * 512), off)
+#define P_ISSUE(kt, Ah, Al, Bh, Bl)                                                        \
+  do {                                                                                     \
+    const bool kok_ = kl + (long)(kt) * P_BK < kend;                                       \
+    const unsigned kb_ = (unsigned)((kt) * P_BK * 2);                                      \
+    const unsigned a0_ = (kok_ && oA0 != nbA) ? oA0 + kb_ : nbA;                          \
+    const unsigned a1_ = (kok_ && oA1 != nbA) ? oA1 + kb_ : nbA;                          \
+    const unsigned a2_ = (kok_ && oA2 != nbA) ? oA2 + kb_ : nbA;                          \
+    const unsigned a3_ = (kok_ && oA3 != nbA) ? oA3 + kb_ : nbA;                          \
+    const unsigned b0_ = (kok_ && oB0 != nbB) ? oB0 + kb_ : nbB;                          \
+    const unsigned b1_ = (kok_ && oB1 != nbB) ? oB1 + kb_ : nbB;                          \
+    P_LD(rah, Ah, wave * 4 + 0, a0_); P_LD(ral, Al, wave * 4 + 0, a0_);                   \
+    P_LD(rah, Ah, wave * 4 + 1, a1_); P_LD(ral, Al, wave * 4 + 1, a1_);                   \
+    P_LD(rah, Ah, wave * 4 + 2, a2_); P_LD(ral, Al, wave * 4 + 2, a2_);                   \
+    P_LD(rah, Ah, wave * 4 + 3, a3_); P_LD(ral, Al, wave * 4 + 3, a3_);                   \
+    P_LD(rbh, Bh, wave * 2 + 0, b0_); P_LD(rbl, Bl, wave * 2 + 0, b0_);                   \
+    P_LD(rbh, Bh, wave * 2 + 1, b1_); P_LD(rbl, Bl, wave * 2 + 1, b1_);                   \
+  } while (0)
+
+  f4_t acc[8][4];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (f4_t)(0.0f);
+
+  // fragment reads: row r = 16 q + (lane & 15) holds k-chunk c = lane >> 4 in slot c ^ ((lane >> 2) & 3)
+  const int slot = ((lane >> 4) ^ ((lane >> 2) & 3)) * 8;
+  const int arow = (wm * 128 + (lane & 15)) * P_BK + slot;
+  const int brow = (wn * 64 + (lane & 15)) * P_BK + slot;
+#define P_COMPUTE(Ah, Al, Bh, Bl)                                                                   \
+  do {                                                                                              \
+    bf8_t ah_[8], al_[8], bh_[4], bl_[4];                                                           \
+    _Pragma("unroll") for (int i = 0; i < 8; ++i) {                                                 \
+      ah_[i] = *(const bf8_t*)((Ah) + arow + i * 16 * P_BK);                                        \
+      al_[i] = *(const bf8_t*)((Al) + arow + i * 16 * P_BK);                                        \
+    }                                                                                               \
+    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                                 \
+      bh_[j] = *(const bf8_t*)((Bh) + brow + j * 16 * P_BK);                                        \
+      bl_[j] = *(const bf8_t*)((Bl) + brow + j * 16 * P_BK);                                        \
+    }                                                                                               \
+    _Pragma("unroll") for (int i = 0; i < 8; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j) {   \
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al_[i], bh_[j], acc[i][j], 0, 0, 0);      \
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah_[i], bl_[j], acc[i][j], 0, 0, 0);      \
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah_[i], bh_[j], acc[i][j], 0, 0, 0);      \
+    }                                                                                               \
+  } while (0)
+
+  const int nk = (kend > kbeg) ? (int)((kend - kbeg + P_BK - 1) / P_BK) : 0;
+  if (nk > 0) {
+    P_ISSUE(0, a0h, a0l, b0h, b0l);
+    if (nk > 1) P_ISSUE(1, a1h, a1l, b1h, b1l);
+    for (int kt = 0; kt < nk; kt += 3) {
+      // stage kt (ring slot 0): its loads are done once at most the next stage's 12 remain
+      if (kt + 1 < nk) P_WAIT_BARRIER(12); else P_WAIT_BARRIER(0);
+      if (kt + 2 < nk) P_ISSUE(kt + 2, a2h, a2l, b2h, b2l);
+      P_COMPUTE(a0h, a0l, b0h, b0l);
+      if (kt + 1 >= nk) break;
+      if (kt + 2 < nk) P_WAIT_BARRIER(12); else P_WAIT_BARRIER(0);
+      if (kt + 3 < nk) P_ISSUE(kt + 3, a0h, a0l, b0h, b0l);
+      P_COMPUTE(a1h, a1l, b1h, b1l);
+      if (kt + 2 >= nk) break;
+      if (kt + 3 < nk) P_WAIT_BARRIER(12); else P_WAIT_BARRIER(0);
+      if (kt + 4 < nk) P_ISSUE(kt + 4, a1h, a1l, b1h, b1l);
+      P_COMPUTE(a2h, a2l, b2h, b2l);
+    }
+  }
+#undef P_LD
+#undef P_ISSUE
+#undef P_COMPUTE
+
+  // epilogue: one straight-line pass per output type keeps the accumulators in registers
+  const long cbase = b * p.sC + (long)ks * p.sCk;
+  const float alpha = (float)p.alpha;
+  auto addr = [&](long row, long col) -> long {
+    return cbase + ((p.cdiv > 0) ? (col / p.cdiv) * p.cstr + (col % p.cdiv) * p.cs2 + row * p.ldc : row * p.ldc + col);
+  };
+#define P_EPI(STMT)                                                            \
+  _Pragma("unroll") for (int i = 0; i < 8; ++i)                                \
+  _Pragma("unroll") for (int j = 0; j < 4; ++j)                                \
+  _Pragma("unroll") for (int r = 0; r < 4; ++r) {                              \
+    const long row = m0 + wm * 128 + i * 16 + 4 * (lane >> 4) + r;             \
+    const long col = n0 + wn * 64 + j * 16 + (lane & 15);                      \
+    if (row < M && col < N) {                                                  \
+      const long off = addr(row, col);                                         \
+      const float val = alpha * acc[i][j][r];                                  \
+      STMT;                                                                    \
+    }                                                                          \
+  }
+  if (p.out_type == GEMM_OUT_F64) {
+    double* C = (double*)p.C;
+    if (p.beta != 0.0) {
+      const double beta = p.beta;
+      P_EPI(C[off] = (double)val + beta * C[off]);
+    } else {
+      P_EPI(C[off] = (double)val);
+    }
+  } else if (p.out_type == GEMM_OUT_F32) {
+    float* C = (float*)p.C;
+    const float beta = (float)p.beta;
+    P_EPI(C[off] = beta != 0.0f ? val + beta * C[off] : val);
+  } else {
+    u16* Ch = (u16*)p.C;
+    u16* Cl = (u16*)p.C2;
+    P_EPI(const u16 h = bf16_bits_rne(val); Ch[off] = h; Cl[off] = bf16_bits_rne(val - bf16_bits_to_float(h)));
+  }
+#undef P_EPI
+}
+
 int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st) {
   if (p.a_trans || !p.b_trans) return DFTB_ERR_ARG;
   if (!p.A2 || !p.B2) return DFTB_ERR_ARG;
@@ -170,7 +363,14 @@ int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st) {
   if (p.ksplit > 1 && (p.kchunk & 7)) return DFTB_ERR_ARG;
   if ((((uintptr_t)p.A) | ((uintptr_t)p.A2) | ((uintptr_t)p.B) | ((uintptr_t)p.B2)) & 15) return DFTB_ERR_ARG;
   const bool small = p.tile == 64 || (p.tile == 0 && (p.M < 128 || p.N < 128));
-  if (small) {
+  const bool pipe = p.tile == 256 || (p.tile == 0 && p.M >= 256 && p.N >= 128 && p.K >= 4 * P_BK);
+  if (pipe) {
+    // 32-bit buffer offsets: each workgroup's operand span must stay below 4 GiB
+    const long span_a = ((long)P_BM * p.lda + p.K) * 2, span_b = ((long)P_BN * p.ldb + p.K) * 2;
+    if (span_a >= 0xffff0000L || span_b >= 0xffff0000L) return DFTB_ERR_ARG;
+    dim3 grid(ceil_div(p.N, P_BN), ceil_div(p.M, P_BM), p.batch * p.ksplit);
+    hipLaunchKernelGGL(gemm_bf3_pipe_kernel, grid, dim3(256), 0, st, p);
+  } else if (small) {
     dim3 grid(ceil_div(p.N, 64), ceil_div(p.M, 64), p.batch * p.ksplit);
     hipLaunchKernelGGL((gemm_bf3_kernel<64, 64>), grid, dim3(256), 0, st, p);
   } else {

@@ -115,6 +115,75 @@ def test_gemm_bf16x3(M, N, K, rng):
     assert err < 3e-5 * np.sqrt(K), err
 
 
+def _bf_pair(X):
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _p, _stream
+    hi = torch.empty(X.shape, dtype=torch.int16, device=_dev())
+    lo = torch.empty(X.shape, dtype=torch.int16, device=_dev())
+    _lib.call("dftb_convert_bf16pair", _p(X), X.numel(), _p(hi), _p(lo), _stream())
+    return hi, lo
+
+
+@pytest.mark.parametrize("M,N,K,batch,tile", [(256, 128, 128, 1, 256), (300, 260, 200, 2, 256), (513, 300, 1000, 3, 0),
+                                              (260, 129, 136, 1, 0)])
+def test_gemm_bf16x3_pipelined(M, N, K, batch, tile, rng):
+    """Large-tile pipelined BF16x3 kernel: partial tiles in M, N and K (K tail not a
+    multiple of 32 loads zeros), strided batches, beta accumulation."""
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _stream
+    lda, ldb = K + 8, K + 16
+    A = rng.standard_normal((batch, M, lda))
+    B = rng.standard_normal((batch, N, ldb))
+    C0 = rng.standard_normal((batch, M, N))
+    (ah, al), (bh, bl) = _bf_pair(T(A)), _bf_pair(T(B))
+    C = T(C0.copy())
+    g = _lib.GemmArgs()
+    g.A, g.A2, g.B, g.B2, g.C = ah.data_ptr(), al.data_ptr(), bh.data_ptr(), bl.data_ptr(), C.data_ptr()
+    g.lda, g.ldb, g.ldc = lda, ldb, N
+    g.sA, g.sB, g.sC = M * lda, N * ldb, M * N
+    g.cs2 = 1
+    g.M, g.N, g.K, g.batch, g.ksplit = M, N, K, batch, 1
+    g.alpha, g.beta = 0.5, 1.0
+    g.b_trans = 1
+    g.tile = tile
+    g.in_type, g.out_type = _lib.GEMM_IN_BF16X3, _lib.GEMM_OUT_F64
+    _lib.call("dftb_gemm", ctypes.byref(g), _stream())
+    ref = 0.5 * np.einsum("bmk,bnk->bmn", A[:, :, :K], B[:, :, :K]) + C0
+    err = np.abs(C.cpu().numpy() - ref).max()
+    assert err < 3e-5 * np.sqrt(K), err
+
+
+def test_gemm_bf16x3_pipelined_remap_pair_output(rng):
+    """Column-remapped bf16-pair output (the exchange 'expand' form) of the pipelined kernel."""
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _stream
+    M, K, cdiv, groups = 264, 160, 48, 4          # N = groups * cdiv columns -> out[g][c][row]
+    N = groups * cdiv
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((N, K))
+    (ah, al), (bh, bl) = _bf_pair(T(A)), _bf_pair(T(B))
+    oh = torch.zeros(groups * cdiv * M, dtype=torch.int16, device=_dev())
+    ol = torch.zeros_like(oh)
+    g = _lib.GemmArgs()
+    g.A, g.A2, g.B, g.B2, g.C, g.C2 = (ah.data_ptr(), al.data_ptr(), bh.data_ptr(), bl.data_ptr(), oh.data_ptr(),
+                                       ol.data_ptr())
+    g.lda, g.ldb, g.ldc = K, K, 1
+    g.cdiv, g.cstr, g.cs2 = cdiv, M, groups * M     # col n -> (n // cdiv) * M + (n % cdiv) * groups * M + row
+    g.M, g.N, g.K, g.batch, g.ksplit = M, N, K, 1, 1
+    g.alpha, g.beta = 1.0, 0.0
+    g.b_trans = 1
+    g.tile = 256
+    g.in_type, g.out_type = _lib.GEMM_IN_BF16X3, _lib.GEMM_OUT_BF16PAIR
+    _lib.call("dftb_gemm", ctypes.byref(g), _stream())
+    f = lambda t: (t.to(torch.int32) & 0xFFFF).cpu().numpy().astype(np.uint32) << 16
+    got = f(oh).view(np.float32).astype(np.float64) + f(ol).view(np.float32).astype(np.float64)
+    got = got.reshape(cdiv, groups, M)                 # [n % cdiv][n // cdiv][row]
+    ref = A @ B.T                                      # [row][n]
+    ref = ref.reshape(M, groups, cdiv).transpose(2, 1, 0)
+    # product error plus the rounding of the stored hi/lo pair (~16 significant bits)
+    assert (np.abs(got - ref) <= 3e-5 * np.sqrt(K) + 2.0 ** -15 * np.abs(ref)).all()
+
+
 def test_mfma_f64_lane_layout():
     """A rank-one product with distinct row/column scales pins the MFMA output layout."""
     from dftbaby_amd.engine import gemm
