@@ -177,7 +177,6 @@ __global__ __launch_bounds__(256) void gemm_bf3_kernel(GemmArgs p) {
 #define P_BM 256
 #define P_BN 128
 #define P_BK 32
-#define P_LOADS 12  // buffer->LDS instructions per thread per stage (A hi/lo 4+4, B hi/lo 2+2)
 
 typedef int i4_t __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) u16 lds_u16_t;
@@ -185,10 +184,10 @@ typedef __attribute__((address_space(3))) u16 lds_u16_t;
 // raw buffer descriptor (stride 0, 32-bit data format) over [base, base + bytes)
 __device__ __forceinline__ i4_t make_rsrc(const void* base, unsigned bytes) {
   const unsigned long long a = (unsigned long long)base;
-  i4_t r;
-  r.x = (int)(a & 0xffffffffu);
-  r.y = (int)((a >> 32) & 0xffffu);
-  r.z = (int)bytes;
+  i4_t r;  // workgroup-uniform: pinned to scalar registers for the "s" operand of dma16
+  r.x = __builtin_amdgcn_readfirstlane((int)(a & 0xffffffffu));
+  r.y = __builtin_amdgcn_readfirstlane((int)((a >> 32) & 0xffffu));
+  r.z = __builtin_amdgcn_readfirstlane((int)bytes);
   r.w = 0x00020000;
   return r;
 }
@@ -203,13 +202,20 @@ __device__ __forceinline__ void dma16(i4_t rs, unsigned lds, unsigned off) {
                : "memory");
 }
 
-// wait until at most N buffer loads of this wave are outstanding, then barrier
-#define P_WAIT_BARRIER(N) asm volatile("s_waitcnt vmcnt(" #N ")\n\ts_barrier" ::: "memory")
+// wait until at most N buffer loads of this wave are outstanding and its LDS
+// reads have landed, then barrier
+#define P_WAIT_BARRIER(N) asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory")
 
-__global__ __launch_bounds__(256) void gemm_bf3_pipe_kernel(GemmArgs p) {
-  __shared__ __attribute__((aligned(1024))) u16 a0h[P_BM * P_BK], a0l[P_BM * P_BK], b0h[P_BN * P_BK], b0l[P_BN * P_BK];
-  __shared__ __attribute__((aligned(1024))) u16 a1h[P_BM * P_BK], a1l[P_BM * P_BK], b1h[P_BN * P_BK], b1l[P_BN * P_BK];
-  __shared__ __attribute__((aligned(1024))) u16 a2h[P_BM * P_BK], a2l[P_BM * P_BK], b2h[P_BN * P_BK], b2l[P_BN * P_BK];
+// NW wave64s per workgroup: NW / 2 along M, 2 along N; wave tile (256 / (NW / 2)) x 64.
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
+  constexpr int WMW = NW / 2;                // waves along M
+  constexpr int TM = P_BM / WMW / 16;        // MFMA tiles per wave along M (8 or 4)
+  constexpr int CA = 16 / NW, CB = 8 / NW;   // 1 KB DMA chunks per wave per A / B array
+  constexpr int LOADS = 2 * CA + 2 * CB;     // DMA instructions per wave per stage
+  // ring of three stages, each [A hi | A lo | B hi | B lo] (48 KiB)
+  constexpr int SA = P_BM * P_BK, SB = P_BN * P_BK, STAGE = 2 * SA + 2 * SB;
+  __shared__ __attribute__((aligned(1024))) u16 ring[3 * STAGE];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: the LDS destination goes to M0
   const int wm = wave >> 1, wn = wave & 1;
@@ -232,88 +238,89 @@ __global__ __launch_bounds__(256) void gemm_bf3_pipe_kernel(GemmArgs p) {
   const i4_t rbl = make_rsrc((const u16*)p.B2 + b * p.sB + n0 * p.ldb, nbB);
   // this lane's chunk: row 16 * ch + lane / 4 of the tile, k-chunk (lane & 3) ^ ((lane >> 4) & 3)
   const int cl = (lane & 3) ^ ((lane >> 4) & 3);
-  unsigned oA0, oA1, oA2, oA3, oB0, oB1;
+  unsigned oA[CA], oB[CB];
   {
     const long r = lane >> 2;
-    auto offa = [&](int i) -> unsigned {
-      const long rr = (wave * 4 + i) * 16 + r;
-      return rr < mrows ? (unsigned)((rr * p.lda + kbeg + 8 * cl) * 2) : nbA;
-    };
-    auto offb = [&](int i) -> unsigned {
-      const long rr = (wave * 2 + i) * 16 + r;
-      return rr < nrows ? (unsigned)((rr * p.ldb + kbeg + 8 * cl) * 2) : nbB;
-    };
-    oA0 = offa(0); oA1 = offa(1); oA2 = offa(2); oA3 = offa(3);
-    oB0 = offb(0); oB1 = offb(1);
+#pragma unroll
+    for (int i = 0; i < CA; ++i) {
+      const long rr = (wave * CA + i) * 16 + r;
+      oA[i] = rr < mrows ? (unsigned)((rr * p.lda + kbeg + 8 * cl) * 2) : nbA;
+    }
+#pragma unroll
+    for (int i = 0; i < CB; ++i) {
+      const long rr = (wave * CB + i) * 16 + r;
+      oB[i] = rr < nrows ? (unsigned)((rr * p.ldb + kbeg + 8 * cl) * 2) : nbB;
+    }
   }
   const long kl = kbeg + 8 * cl;  // k of this lane's chunk at stage 0
 
 #define P_LD(rs, base, ch, off) dma16(rs, lds_addr((base) + (ch) * 512), off)
-#define P_ISSUE(kt, Ah, Al, Bh, Bl)                                                        \
-  do {                                                                                     \
-    const bool kok_ = kl + (long)(kt) * P_BK < kend;                                       \
-    const unsigned kb_ = (unsigned)((kt) * P_BK * 2);                                      \
-    const unsigned a0_ = (kok_ && oA0 != nbA) ? oA0 + kb_ : nbA;                          \
-    const unsigned a1_ = (kok_ && oA1 != nbA) ? oA1 + kb_ : nbA;                          \
-    const unsigned a2_ = (kok_ && oA2 != nbA) ? oA2 + kb_ : nbA;                          \
-    const unsigned a3_ = (kok_ && oA3 != nbA) ? oA3 + kb_ : nbA;                          \
-    const unsigned b0_ = (kok_ && oB0 != nbB) ? oB0 + kb_ : nbB;                          \
-    const unsigned b1_ = (kok_ && oB1 != nbB) ? oB1 + kb_ : nbB;                          \
-    P_LD(rah, Ah, wave * 4 + 0, a0_); P_LD(ral, Al, wave * 4 + 0, a0_);                   \
-    P_LD(rah, Ah, wave * 4 + 1, a1_); P_LD(ral, Al, wave * 4 + 1, a1_);                   \
-    P_LD(rah, Ah, wave * 4 + 2, a2_); P_LD(ral, Al, wave * 4 + 2, a2_);                   \
-    P_LD(rah, Ah, wave * 4 + 3, a3_); P_LD(ral, Al, wave * 4 + 3, a3_);                   \
-    P_LD(rbh, Bh, wave * 2 + 0, b0_); P_LD(rbl, Bl, wave * 2 + 0, b0_);                   \
-    P_LD(rbh, Bh, wave * 2 + 1, b1_); P_LD(rbl, Bl, wave * 2 + 1, b1_);                   \
+#define P_ISSUE(kt, Ah, Al, Bh, Bl)                                                  \
+  do {                                                                               \
+    const bool kok_ = kl + (long)(kt) * P_BK < kend;                                 \
+    const unsigned kb_ = (unsigned)((kt) * P_BK * 2);                                \
+    _Pragma("unroll") for (int i = 0; i < CA; ++i) {                                 \
+      const unsigned o_ = (kok_ && oA[i] != nbA) ? oA[i] + kb_ : nbA;                \
+      P_LD(rah, Ah, wave * CA + i, o_);                                              \
+      P_LD(ral, Al, wave * CA + i, o_);                                              \
+    }                                                                                \
+    _Pragma("unroll") for (int i = 0; i < CB; ++i) {                                 \
+      const unsigned o_ = (kok_ && oB[i] != nbB) ? oB[i] + kb_ : nbB;                \
+      P_LD(rbh, Bh, wave * CB + i, o_);                                              \
+      P_LD(rbl, Bl, wave * CB + i, o_);                                              \
+    }                                                                                \
   } while (0)
 
-  f4_t acc[8][4];
+  f4_t acc[TM][4];
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
+  for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (f4_t)(0.0f);
 
   // fragment reads: row r = 16 q + (lane & 15) holds k-chunk c = lane >> 4 in slot c ^ ((lane >> 2) & 3)
   const int slot = ((lane >> 4) ^ ((lane >> 2) & 3)) * 8;
-  const int arow = (wm * 128 + (lane & 15)) * P_BK + slot;
+  const int arow = (wm * (TM * 16) + (lane & 15)) * P_BK + slot;
   const int brow = (wn * 64 + (lane & 15)) * P_BK + slot;
-#define P_COMPUTE(Ah, Al, Bh, Bl)                                                                   \
-  do {                                                                                              \
-    bf8_t ah_[8], al_[8], bh_[4], bl_[4];                                                           \
-    _Pragma("unroll") for (int i = 0; i < 8; ++i) {                                                 \
-      ah_[i] = *(const bf8_t*)((Ah) + arow + i * 16 * P_BK);                                        \
-      al_[i] = *(const bf8_t*)((Al) + arow + i * 16 * P_BK);                                        \
-    }                                                                                               \
-    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                                 \
-      bh_[j] = *(const bf8_t*)((Bh) + brow + j * 16 * P_BK);                                        \
-      bl_[j] = *(const bf8_t*)((Bl) + brow + j * 16 * P_BK);                                        \
-    }                                                                                               \
-    _Pragma("unroll") for (int i = 0; i < 8; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j) {   \
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al_[i], bh_[j], acc[i][j], 0, 0, 0);      \
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah_[i], bl_[j], acc[i][j], 0, 0, 0);      \
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah_[i], bh_[j], acc[i][j], 0, 0, 0);      \
-    }                                                                                               \
+#define P_COMPUTE(Ah, Al, Bh, Bl)                                                                    \
+  do {                                                                                               \
+    bf8_t ah_[TM], al_[TM], bh_[4], bl_[4];                                                          \
+    _Pragma("unroll") for (int i = 0; i < TM; ++i) {                                                 \
+      ah_[i] = *(const bf8_t*)((Ah) + arow + i * 16 * P_BK);                                         \
+      al_[i] = *(const bf8_t*)((Al) + arow + i * 16 * P_BK);                                         \
+    }                                                                                                \
+    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                                  \
+      bh_[j] = *(const bf8_t*)((Bh) + brow + j * 16 * P_BK);                                         \
+      bl_[j] = *(const bf8_t*)((Bl) + brow + j * 16 * P_BK);                                         \
+    }                                                                                                \
+    /* product-major order: independent MFMAs separate two updates of one accumulator */            \
+    _Pragma("unroll") for (int i = 0; i < TM; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j)     \
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al_[i], bh_[j], acc[i][j], 0, 0, 0);       \
+    _Pragma("unroll") for (int i = 0; i < TM; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j)     \
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah_[i], bl_[j], acc[i][j], 0, 0, 0);       \
+    _Pragma("unroll") for (int i = 0; i < TM; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j)     \
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah_[i], bh_[j], acc[i][j], 0, 0, 0);       \
   } while (0)
 
   const int nk = (kend > kbeg) ? (int)((kend - kbeg + P_BK - 1) / P_BK) : 0;
+#define P_SLOT(st) (ring + (st) * STAGE)
   if (nk > 0) {
-    P_ISSUE(0, a0h, a0l, b0h, b0l);
-    if (nk > 1) P_ISSUE(1, a1h, a1l, b1h, b1l);
-    for (int kt = 0; kt < nk; kt += 3) {
-      // stage kt (ring slot 0): its loads are done once at most the next stage's 12 remain
-      if (kt + 1 < nk) P_WAIT_BARRIER(12); else P_WAIT_BARRIER(0);
-      if (kt + 2 < nk) P_ISSUE(kt + 2, a2h, a2l, b2h, b2l);
-      P_COMPUTE(a0h, a0l, b0h, b0l);
-      if (kt + 1 >= nk) break;
-      if (kt + 2 < nk) P_WAIT_BARRIER(12); else P_WAIT_BARRIER(0);
-      if (kt + 3 < nk) P_ISSUE(kt + 3, a0h, a0l, b0h, b0l);
-      P_COMPUTE(a1h, a1l, b1h, b1l);
-      if (kt + 2 >= nk) break;
-      if (kt + 3 < nk) P_WAIT_BARRIER(12); else P_WAIT_BARRIER(0);
-      if (kt + 4 < nk) P_ISSUE(kt + 4, a1h, a1l, b1h, b1l);
-      P_COMPUTE(a2h, a2l, b2h, b2l);
+    { u16* S0 = P_SLOT(0); P_ISSUE(0, S0, S0 + SA, S0 + 2 * SA, S0 + 2 * SA + SB); }
+    if (nk > 1) { u16* S1 = P_SLOT(1); P_ISSUE(1, S1, S1 + SA, S1 + 2 * SA, S1 + 2 * SA + SB); }
+    int cur = 0;
+    for (int kt = 0; kt < nk; ++kt) {
+      // stage kt is complete once at most the next stage's loads remain in flight
+      if (kt + 1 < nk) P_WAIT_BARRIER(LOADS); else P_WAIT_BARRIER(0);
+      if (kt + 2 < nk) {
+        const int nx = cur == 0 ? 2 : cur - 1;   // slot of stage kt + 2 = (kt + 2) % 3
+        u16* S = P_SLOT(nx);
+        P_ISSUE(kt + 2, S, S + SA, S + 2 * SA, S + 2 * SA + SB);
+      }
+      const u16* C = P_SLOT(cur);
+      P_COMPUTE(C, C + SA, C + 2 * SA, C + 2 * SA + SB);
+      cur = cur == 2 ? 0 : cur + 1;
     }
   }
+#undef P_SLOT
 #undef P_LD
 #undef P_ISSUE
 #undef P_COMPUTE
@@ -325,10 +332,10 @@ __global__ __launch_bounds__(256) void gemm_bf3_pipe_kernel(GemmArgs p) {
     return cbase + ((p.cdiv > 0) ? (col / p.cdiv) * p.cstr + (col % p.cdiv) * p.cs2 + row * p.ldc : row * p.ldc + col);
   };
 #define P_EPI(STMT)                                                            \
-  _Pragma("unroll") for (int i = 0; i < 8; ++i)                                \
+  _Pragma("unroll") for (int i = 0; i < TM; ++i)                               \
   _Pragma("unroll") for (int j = 0; j < 4; ++j)                                \
   _Pragma("unroll") for (int r = 0; r < 4; ++r) {                              \
-    const long row = m0 + wm * 128 + i * 16 + 4 * (lane >> 4) + r;             \
+    const long row = m0 + wm * (TM * 16) + i * 16 + 4 * (lane >> 4) + r;       \
     const long col = n0 + wn * 64 + j * 16 + (lane & 15);                      \
     if (row < M && col < N) {                                                  \
       const long off = addr(row, col);                                         \
@@ -356,6 +363,11 @@ __global__ __launch_bounds__(256) void gemm_bf3_pipe_kernel(GemmArgs p) {
 #undef P_EPI
 }
 
+// wave count of the pipelined kernel (8: two waves per SIMD hide each other's stalls)
+#ifndef DFTB_BF3_WAVES
+#define DFTB_BF3_WAVES 8
+#endif
+
 int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st) {
   if (p.a_trans || !p.b_trans) return DFTB_ERR_ARG;
   if (!p.A2 || !p.B2) return DFTB_ERR_ARG;
@@ -369,7 +381,7 @@ int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st) {
     const long span_a = ((long)P_BM * p.lda + p.K) * 2, span_b = ((long)P_BN * p.ldb + p.K) * 2;
     if (span_a >= 0xffff0000L || span_b >= 0xffff0000L) return DFTB_ERR_ARG;
     dim3 grid(ceil_div(p.N, P_BN), ceil_div(p.M, P_BM), p.batch * p.ksplit);
-    hipLaunchKernelGGL(gemm_bf3_pipe_kernel, grid, dim3(256), 0, st, p);
+    hipLaunchKernelGGL(gemm_bf3_pipe_kernel<DFTB_BF3_WAVES>, grid, dim3(DFTB_BF3_WAVES * 64), 0, st, p);
   } else if (small) {
     dim3 grid(ceil_div(p.N, 64), ceil_div(p.M, 64), p.batch * p.ksplit);
     hipLaunchKernelGGL((gemm_bf3_kernel<64, 64>), grid, dim3(256), 0, st, p);
