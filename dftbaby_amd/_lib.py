@@ -30,6 +30,7 @@ class GemmArgs(ctypes.Structure):
                 ("lda", _l), ("ldb", _l), ("ldc", _l),
                 ("sA", _l), ("sB", _l), ("sC", _l),
                 ("cdiv", _l), ("cstr", _l), ("cs2", _l),
+                ("rdiv", _l), ("rstr", _l), ("rs2", _l),
                 ("M", _i), ("N", _i), ("K", _i),
                 ("batch", _i),
                 ("ksplit", _i), ("kchunk", _i),

@@ -7,6 +7,13 @@ int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st);
 int gemm_reduce_launch(const double* P, long sP, int nsplit, double* C, long n, double alpha, double beta,
                        hipStream_t st);
 
+// Output offset of element (row, col) within one batch entry (column and row remaps).
+__device__ __forceinline__ long gemm_out_offset(const GemmArgs& p, long row, long col) {
+  const long r = (p.rdiv > 0) ? (row / p.rdiv) * p.rstr + (row % p.rdiv) * p.rs2 : row * p.ldc;
+  const long c = (p.cdiv > 0) ? (col / p.cdiv) * p.cstr + (col % p.cdiv) * p.cs2 : col;
+  return r + c;
+}
+
 static inline GemmArgs gemm_args_default() {
   GemmArgs p;
   p.A = p.A2 = p.B = p.B2 = nullptr;
@@ -16,6 +23,9 @@ static inline GemmArgs gemm_args_default() {
   p.cdiv = 0;
   p.cstr = 0;
   p.cs2 = 1;
+  p.rdiv = 0;
+  p.rstr = 0;
+  p.rs2 = 0;
   p.M = p.N = p.K = 0;
   p.batch = 1;
   p.ksplit = 1;

@@ -13,10 +13,11 @@
 // (double buffered, one barrier per K step) so that the 16 lanes that feed
 // one MFMA row/column read 16 consecutive LDS words.
 //
-// Output modes: plain row-major, or "column remap" where output column n is
-// split as (n / cdiv, n % cdiv) and written to (n / cdiv) * cstr +
-// (n % cdiv) * cs2 + row * ldc.  The remap writes the exchange intermediates directly in the
-// [atom][row][col] order the following contraction consumes.  Split-K writes
+// Output modes: plain row-major, or a column and/or row remap where output
+// column n is split as (n / cdiv, n % cdiv) and written at (n / cdiv) * cstr +
+// (n % cdiv) * cs2 (rows likewise with rdiv, rstr, rs2).  The remap writes the
+// exchange intermediates directly in the order the following contraction
+// consumes.  Split-K writes
 // per-slice partial results at C + slice * sCk (reduced by gemm_reduce).
 #include "common.h"
 #include "gemm.h"
@@ -35,8 +36,7 @@ __device__ __forceinline__ f4_t mfma16(float a, float b, f4_t c) {
 }
 
 __device__ __forceinline__ long remap_addr(const GemmArgs& p, long row, long col) {
-  if (p.cdiv > 0) return (col / p.cdiv) * p.cstr + (col % p.cdiv) * p.cs2 + row * p.ldc;
-  return row * p.ldc + col;
+  return gemm_out_offset(p, row, col);
 }
 
 __device__ __forceinline__ void store_out(const GemmArgs& p, long off, double val) {

@@ -155,11 +155,7 @@ __global__ __launch_bounds__(256) void gemm_bf3_kernel(GemmArgs p) {
       for (int r = 0; r < 4; ++r) {
         long row = m0 + wm * (BM / 2) + i * 16 + 4 * (lane >> 4) + r;
         long col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
-        if (row < M && col < N) {
-          long off = (p.cdiv > 0) ? (col / p.cdiv) * p.cstr + (col % p.cdiv) * p.cs2 + row * p.ldc
-                                  : row * p.ldc + col;
-          store_out_bf(p, cbase + off, alpha * acc[i][j][r]);
-        }
+        if (row < M && col < N) store_out_bf(p, cbase + gemm_out_offset(p, row, col), alpha * acc[i][j][r]);
       }
 }
 
@@ -328,9 +324,7 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
   // epilogue: one straight-line pass per output type keeps the accumulators in registers
   const long cbase = b * p.sC + (long)ks * p.sCk;
   const float alpha = (float)p.alpha;
-  auto addr = [&](long row, long col) -> long {
-    return cbase + ((p.cdiv > 0) ? (col / p.cdiv) * p.cstr + (col % p.cdiv) * p.cs2 + row * p.ldc : row * p.ldc + col);
-  };
+  auto addr = [&](long row, long col) -> long { return cbase + gemm_out_offset(p, row, col); };
 #define P_EPI(STMT)                                                            \
   _Pragma("unroll") for (int i = 0; i < TM; ++i)                               \
   _Pragma("unroll") for (int j = 0; j < 4; ++j)                                \

@@ -331,15 +331,18 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
     const unsigned short* VTl = VTh + total;
     for (int A0 = 0; prec == 2 && A0 < nat_eff; A0 += L.ac) {
       const int ac = (nat_eff - A0 < L.ac) ? (nat_eff - A0) : L.ac;
+      // K1 expand with the slice as the row operand: rows (a', a) of qf_vv, columns j
+      // of V; row (a', a) is stored at a * ac * opad + a' * opad, so the 16 lanes of
+      // one MFMA row group write 16 consecutive j
       GemmArgs e = gemm_args_default();
       e.in_type = GEMM_IN_BF16X3; e.out_type = GEMM_OUT_BF16PAIR;
-      e.A = Vx; e.A2 = Vlo; e.lda = vpad; e.sA = L.N;
-      e.B = (const u16_t*)op->qf_vv[0] + A0 * vpad * vpad;
-      e.B2 = (const u16_t*)op->qf_vv[1] + A0 * vpad * vpad;
-      e.b_trans = 1; e.ldb = vpad; e.sB = 0;
-      e.C = T; e.C2 = Tlo; e.cdiv = vpad; e.cstr = opad; e.cs2 = (long)ac * opad; e.ldc = 1;
+      e.A = (const u16_t*)op->qf_vv[0] + A0 * vpad * vpad;
+      e.A2 = (const u16_t*)op->qf_vv[1] + A0 * vpad * vpad;
+      e.lda = vpad; e.sA = 0;
+      e.B = Vx; e.B2 = Vlo; e.b_trans = 1; e.ldb = vpad; e.sB = L.N;
+      e.C = T; e.C2 = Tlo; e.rdiv = vpad; e.rstr = opad; e.rs2 = (long)ac * opad;
       e.sC = (long)ac * opad * vpad;
-      e.M = (int)opad; e.N = (int)(ac * vpad); e.K = (int)vpad; e.batch = nvec; e.tile = op->tile_expand;
+      e.M = (int)(ac * vpad); e.N = (int)opad; e.K = (int)vpad; e.batch = nvec; e.tile = op->tile_expand;
       if ((rc = gemm_launch(e, st))) return rc;
       GemmArgs c = gemm_args_default();
       c.in_type = GEMM_IN_BF16X3; c.out_type = GEMM_OUT_F64;
@@ -350,15 +353,16 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
       c.C = X1; c.ldc = vpad; c.sC = L.N; c.beta = 1.0;
       c.M = op->nocc; c.N = (int)vpad; c.K = (int)(ac * opad); c.batch = nvec; c.tile = op->tile_contract;
       if ((rc = gemm_launch(c, st))) return rc;
+      // K2 expand, same orientation: rows (a', a) of qf_ov, columns b of V^T
       GemmArgs e2 = gemm_args_default();
       e2.in_type = GEMM_IN_BF16X3; e2.out_type = GEMM_OUT_BF16PAIR;
-      e2.A = VTh; e2.A2 = VTl; e2.lda = opad; e2.sA = L.N;
-      e2.B = (const u16_t*)op->qf_ov[0] + A0 * vpad * opad;
-      e2.B2 = (const u16_t*)op->qf_ov[1] + A0 * vpad * opad;
-      e2.b_trans = 1; e2.ldb = opad; e2.sB = 0;
-      e2.C = T; e2.C2 = Tlo; e2.cdiv = vpad; e2.cstr = vpad; e2.cs2 = (long)ac * vpad; e2.ldc = 1;
+      e2.A = (const u16_t*)op->qf_ov[0] + A0 * vpad * opad;
+      e2.A2 = (const u16_t*)op->qf_ov[1] + A0 * vpad * opad;
+      e2.lda = opad; e2.sA = 0;
+      e2.B = VTh; e2.B2 = VTl; e2.b_trans = 1; e2.ldb = opad; e2.sB = L.N;
+      e2.C = T; e2.C2 = Tlo; e2.rdiv = vpad; e2.rstr = vpad; e2.rs2 = (long)ac * vpad;
       e2.sC = (long)ac * vpad * vpad;
-      e2.M = (int)vpad; e2.N = (int)(ac * vpad); e2.K = (int)opad; e2.batch = nvec; e2.tile = op->tile_expand;
+      e2.M = (int)(ac * vpad); e2.N = (int)vpad; e2.K = (int)opad; e2.batch = nvec; e2.tile = op->tile_expand;
       if ((rc = gemm_launch(e2, st))) return rc;
       GemmArgs c2 = gemm_args_default();
       c2.in_type = GEMM_IN_BF16X3; c2.out_type = GEMM_OUT_F64;

@@ -25,8 +25,10 @@ extern "C" {
 /* ---- generic batched GEMM ------------------------------------------------
  * C[b] = alpha * op(A[b]) op(B[b]) + beta * C[b] on MFMA.  Element (m,k) of A
  * is A[b*sA + (a_trans ? k*lda + m : m*lda + k)], element (k,n) of B is
- * B[b*sB + (b_trans ? n*ldb + k : k*ldb + n)].  With cdiv > 0 output column n
- * is written to (n / cdiv) * cstr + (n % cdiv) * cs2 + m * ldc.  ksplit > 1 splits K
+ * B[b*sB + (b_trans ? n*ldb + k : k*ldb + n)].  Output element (m, n) goes to
+ * C[b*sC + row(m) + col(n)] with row(m) = m * ldc, or (m / rdiv) * rstr +
+ * (m % rdiv) * rs2 when rdiv > 0, and col(n) = n, or (n / cdiv) * cstr +
+ * (n % cdiv) * cs2 when cdiv > 0.  ksplit > 1 splits K
  * into chunks of kchunk and writes slice s at C + s*sCk (beta must be 0).
  * in_type GEMM_IN_BF16X3 takes (hi, lo) bf16 operand pairs in (A, A2) and
  * (B, B2) and forms hi*hi + hi*lo + lo*hi with FP32 accumulation; it needs
@@ -44,6 +46,7 @@ typedef struct GemmArgs {
   long lda, ldb, ldc;
   long sA, sB, sC;
   long cdiv, cstr, cs2;
+  long rdiv, rstr, rs2;
   int M, N, K;
   int batch;
   int ksplit, kchunk;

@@ -24,7 +24,8 @@ def ground_state(name, lc, sigma=0.0, seed=0):
     if key not in _GS_CACHE:
         builders = {"ethylene": g.ethylene, "benzene": g.benzene,
                     "C8H10": lambda: g.polyene(8), "C12H14": lambda: g.polyene(12),
-                    "C20H22": lambda: g.polyene(20)}
+                    "C20H22": lambda: g.polyene(20),
+                    "C180H36": lambda: g.graphene_flake_armchair((11.5, 11.5, 11.5))}
         atoms = builders[name]()
         if sigma > 0:
             atoms = g.displace(atoms, sigma, seed)

@@ -184,6 +184,35 @@ def test_gemm_bf16x3_pipelined_remap_pair_output(rng):
     assert (np.abs(got - ref) <= 3e-5 * np.sqrt(K) + 2.0 ** -15 * np.abs(ref)).all()
 
 
+def test_gemm_bf16x3_pipelined_row_remap(rng):
+    """Row-remapped bf16-pair output (the swapped exchange 'expand'): row (g, a) -> a * rs2 + g * rstr."""
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _stream
+    rdiv, groups, N, K = 40, 9, 136, 144           # M = groups * rdiv rows
+    M = groups * rdiv
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((N, K))
+    (ah, al), (bh, bl) = _bf_pair(T(A)), _bf_pair(T(B))
+    oh = torch.zeros(M * N, dtype=torch.int16, device=_dev())
+    ol = torch.zeros_like(oh)
+    g = _lib.GemmArgs()
+    g.A, g.A2, g.B, g.B2, g.C, g.C2 = (ah.data_ptr(), al.data_ptr(), bh.data_ptr(), bl.data_ptr(), oh.data_ptr(),
+                                       ol.data_ptr())
+    g.lda, g.ldb = K, K
+    g.rdiv, g.rstr, g.rs2 = rdiv, N, groups * N     # out[a][g][n]
+    g.cs2 = 1
+    g.M, g.N, g.K, g.batch, g.ksplit = M, N, K, 1, 1
+    g.alpha, g.beta = 1.0, 0.0
+    g.b_trans = 1
+    g.tile = 256
+    g.in_type, g.out_type = _lib.GEMM_IN_BF16X3, _lib.GEMM_OUT_BF16PAIR
+    _lib.call("dftb_gemm", ctypes.byref(g), _stream())
+    f = lambda t: (t.to(torch.int32) & 0xFFFF).cpu().numpy().astype(np.uint32) << 16
+    got = (f(oh).view(np.float32).astype(np.float64) + f(ol).view(np.float32).astype(np.float64)).reshape(rdiv, groups, N)
+    ref = (A @ B.T).reshape(groups, rdiv, N).transpose(1, 0, 2)
+    assert (np.abs(got - ref) <= 3e-5 * np.sqrt(K) + 2.0 ** -15 * np.abs(ref)).all()
+
+
 def test_mfma_f64_lane_layout():
     """A rank-one product with distinct row/column scales pins the MFMA output layout."""
     from dftbaby_amd.engine import gemm
@@ -291,6 +320,23 @@ def test_casida_matvec(precision, tol, rng):
     # padding stays zero
     Pp = P.reshape(nvec, op.opad, op.vpad).cpu().numpy()
     assert np.all(Pp[:, o:, :] == 0) and np.all(Pp[:, :, v:] == 0)
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (2, 3e-5)])
+def test_casida_matvec_flake(precision, tol, rng):
+    """Armchair flake C180H36 (o = v = 378): the exchange GEMMs take the large-tile
+    pipelined path (expands with row remap, contracts with M >= 256)."""
+    op, (g, glr, q_oo, q_ov, q_vv, omega) = _operator("C180H36", 1, 0.05, 5)
+    o, v = omega.shape
+    assert o >= 256 and v >= 256
+    nvec = 3
+    vs = rng.standard_normal((o, v, nvec))
+    refP = O.ApBv(g, glr, q_oo, q_ov, q_vv, omega, vs, 1)
+    refM = O.AmBv(g, glr, q_oo, q_ov, q_vv, omega, vs, 1)
+    P, M = op.matvec(op.to_padded(np.moveaxis(vs, 2, 0)), precision, atom_chunk=50)
+    scale = np.abs(refP).max()
+    assert np.abs(op.from_padded(P).cpu().numpy() - np.moveaxis(refP, 2, 0)).max() < tol * scale
+    assert np.abs(op.from_padded(M).cpu().numpy() - np.moveaxis(refM, 2, 0)).max() < tol * scale
 
 
 def test_casida_matvec_atom_chunks(rng):
