@@ -350,6 +350,7 @@ class _Subspace:
 def _subspace_cap(N, l0, k_add, maxiter, nmat, device, frac=0.6):
     want = l0 + k_add * maxiter
     free, total = torch.cuda.mem_get_info(device)
+    free += torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)   # torch's cached blocks
     by_mem = int(frac * free / (8.0 * N * nmat))
     return max(l0 + k_add, min(want, by_mem))
 

@@ -188,13 +188,12 @@ class PairMLP:
 # ---------------------------------------------------------------------------
 # features and targets
 # ---------------------------------------------------------------------------
-def pair_features(op):
-    """[N][6] float32 features on the padded pair grid of a CasidaOperatorGPU."""
+def _diagonal_integrals(op):
+    """(ia|ia), (ia|ia)_lr and (ii|aa)_lr on the padded pair grid (FP64), and ||q^ia||^2."""
     dev = op.device
     st = _stream()
     N, nat = op.N, op.nat
     qc = op.q["c"].reshape(nat, N)
-    # (ia|ia) = q^T gamma q (columnwise), (ia|ia)_lr likewise; ||q||^2
     P = torch.empty((nat, N), dtype=torch.float64, device=dev)
     cc = torch.empty(N, dtype=torch.float64, device=dev)
     cl = torch.empty(N, dtype=torch.float64, device=dev)
@@ -207,14 +206,33 @@ def pair_features(op):
     else:
         cl.zero_()
     call("dftb_coldot", _p(qc), _p(qc), nat, N, _p(qn2), st)
-    # (ii|aa)_lr = q_ii^T gamma_lr q_aa  -> [opad][vpad]
     xl = torch.zeros((op.opad, op.vpad), dtype=torch.float64, device=dev)
     if op.lc:
         X = torch.empty((op.nocc, nat), dtype=torch.float64, device=dev)
         gemm(op.qii, op.gamma_lr, X, op.nocc, nat, nat, nat, nat, nat)
         gemm(X, op.qaa, xl, op.nocc, op.nvirt, nat, nat, nat, op.vpad, b_trans=1)
-    F = torch.empty((N, NFEAT), dtype=torch.float32, device=dev)
-    call("dftb_pair_features", _p(op.omega), _p(qn2), _p(cc), _p(cl), _p(xl.reshape(-1)), N, NFEAT, _p(F), st)
+    return cc, cl, xl.reshape(-1), qn2
+
+
+def casida_diagonal_omega(op):
+    """sqrt((A+B)_ia,ia (A-B)_ia,ia) on real pairs (omega on the padding): the
+    diagonal of the Casida operator, a sharper preconditioner than omega when
+    the Coulomb and long-range exchange diagonals are large."""
+    cc, cl, xl, _ = _diagonal_integrals(op)
+    w = op.omega
+    apb = w + 4.0 * cc - xl - cl
+    amb = w - xl + cl
+    d = torch.sqrt(torch.clamp(apb * amb, min=1e-8))
+    mask = real_mask(op)
+    return torch.where(mask, d, w).contiguous()
+
+
+def pair_features(op):
+    """[N][6] float32 features on the padded pair grid of a CasidaOperatorGPU:
+    omega, ||q^ia||, A_d, B_d, A_d - B_d, sqrt((A+B)_d (A-B)_d)."""
+    cc, cl, xl, qn2 = _diagonal_integrals(op)
+    F = torch.empty((op.N, NFEAT), dtype=torch.float32, device=op.device)
+    call("dftb_pair_features", _p(op.omega), _p(qn2), _p(cc), _p(cl), _p(xl), op.N, NFEAT, _p(F), _stream())
     return F
 
 
@@ -251,18 +269,25 @@ def training_rows(op, Omega, C, n_top=2000, n_rand=6000, seed=0):
 # ---------------------------------------------------------------------------
 # inference: learned initial space + preconditioner omega
 # ---------------------------------------------------------------------------
-def learned_start(op, model, nstates, M=None):
-    """Initial expansion vectors (k x N, padded) and preconditioner omega (N) from the model."""
+def learned_start(op, model, nstates, M=None, use_shift=False, rank="model"):
+    """Initial expansion vectors (2k x N, padded) and preconditioner diagonal (N).
+
+    The M highest-scoring pairs (rank="model": the network's score; "omega":
+    the smallest orbital-energy gaps, no network) span a restricted Casida
+    problem whose lowest k roots give the initial space.  The preconditioner
+    diagonal is the Casida diagonal sqrt((A+B)_d (A-B)_d); use_shift adds the
+    network's second output to it (measured to slow convergence on the
+    graphene flakes, so off by default)."""
     from .engine import Workspace
     dev = op.device
     st = _stream()
     k = int(nstates)
     F = pair_features(op)
-    Y = model.predict(F)
+    Y = model.predict(F) if model is not None else torch.zeros((op.N, NOUT), dtype=torch.float32, device=dev)
     mask = real_mask(op)
-    score = Y[:, 0].clone()
+    score = Y[:, 0].clone() if rank == "model" else -op.omega.to(torch.float32)
     score[~mask] = -1e30
-    M = int(M or max(4 * k, 32))
+    M = int(M or max(50 * k, 256))
     M = min(M, op.nocc * op.nvirt)
     sc = score.cpu().numpy()
     sel = np.argpartition(-sc, M - 1)[:M]
@@ -301,9 +326,17 @@ def learned_start(op, model, nstates, M=None):
     Fv = torch.empty((M, kk), dtype=torch.float64, device=dev)
     w, wn = ws.get(_lib.load().dftb_syevx_workspace(M, kk))
     call("dftb_syevx", _p(R), M, M, kk, _p(w2), _p(Fv), _p(w), wn, st)
+    # X+Y = S F / sqrt(Omega) and X-Y = S^{-1} F sqrt(Omega) of the restricted roots: both
+    # directions seed the non-Hermitian subspace (as the warm start [X-Y; X+Y] does)
     XpY = torch.empty((M, kk), dtype=torch.float64, device=dev)
     gemm(S, Fv, XpY, M, kk, M, M, kk, kk)
-    V = torch.zeros((kk, op.N), dtype=torch.float64, device=dev)
-    V[:, torch.as_tensor(sel, device=dev)] = XpY.T
-    prec_omega = op.omega + Y[:, 1].to(torch.float64) * mask.to(torch.float64)
+    XmY = torch.empty((M, kk), dtype=torch.float64, device=dev)
+    gemm(Si, Fv, XmY, M, kk, M, M, kk, kk)
+    V = torch.zeros((2 * kk, op.N), dtype=torch.float64, device=dev)
+    cols = torch.as_tensor(sel, device=dev)
+    V[:kk, cols] = XmY.T
+    V[kk:, cols] = XpY.T
+    prec_omega = casida_diagonal_omega(op)
+    if use_shift:
+        prec_omega = prec_omega + Y[:, 1].to(torch.float64) * mask.to(torch.float64)
     return V, prec_omega.contiguous(), torch.sqrt(w2.clamp_min(0))
