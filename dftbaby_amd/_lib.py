@@ -60,6 +60,9 @@ class EnsMolecule(ctypes.Structure):
 
 _SIGS = {
     "dftb_gemm": (_i, [ctypes.POINTER(GemmArgs), _vp]),
+    "dftb_gemm_timing_enable": (_i, [_i]),
+    "dftb_gemm_timing_read": (_i, [_i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long),
+                                   ctypes.POINTER(ctypes.c_double)]),
     "dftb_trans_charges_workspace": (_sz, [_i, _i, _i, _i]),
     "dftb_trans_charges": (_i, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _i, _i, _i, _i,
                                 _vp, _vp, _vp, _vp, _sz, _vp]),

@@ -3,6 +3,15 @@
 #include "../../include/dftbaby_amd.h"
 
 int gemm_launch(const GemmArgs& p, hipStream_t st);
+
+// Optional launch timing (see dftb_gemm_timing_*): kernel classes
+enum { GT_F64_128 = 0, GT_F64_64 = 1, GT_F32_128 = 2, GT_F32_64 = 3, GT_BF3_PIPE = 4, GT_BF3_TILE = 5,
+       GT_F64_128_T = 6, GT_NCLASS = 7 };
+struct GemmTimer {
+  int slot;
+  void begin(hipStream_t st);
+  void end(hipStream_t st, int cls, double flops);
+};
 int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st);
 int gemm_reduce_launch(const double* P, long sP, int nsplit, double* C, long n, double alpha, double beta,
                        hipStream_t st);

@@ -186,6 +186,72 @@ static void launch_fp(const GemmArgs& p, hipStream_t st) {
   else hipLaunchKernelGGL((gemm_fp_kernel<T, BM, BN, BK, true, true>), grid, dim3(256), 0, st, p);
 }
 
+// ---------------------------------------------------------------------------
+// Launch timing: when enabled, every GEMM launch is bracketed by two HIP events
+// on its own stream; dftb_gemm_timing_read synchronises them and returns the
+// summed device time, launch count and algorithmic flops (2 M N K per batch
+// entry; BF16x3 counts its three bf16 products) per kernel class.
+#include <vector>
+namespace {
+struct TimedLaunch {
+  hipEvent_t a, b;
+  int cls;
+  double flops;
+};
+bool g_timing = false;
+std::vector<TimedLaunch> g_launches;
+}  // namespace
+
+void GemmTimer::begin(hipStream_t st) {
+  slot = -1;
+  if (!g_timing) return;
+  TimedLaunch t{};
+  if (hipEventCreate(&t.a) != hipSuccess || hipEventCreate(&t.b) != hipSuccess) return;
+  hipEventRecord(t.a, st);
+  g_launches.push_back(t);
+  slot = (int)g_launches.size() - 1;
+}
+
+void GemmTimer::end(hipStream_t st, int cls, double flops) {
+  if (slot < 0) return;
+  g_launches[slot].cls = cls;
+  g_launches[slot].flops = flops;
+  hipEventRecord(g_launches[slot].b, st);
+}
+
+static void timing_clear() {
+  for (auto& t : g_launches) {
+    hipEventDestroy(t.a);
+    hipEventDestroy(t.b);
+  }
+  g_launches.clear();
+}
+
+extern "C" int dftb_gemm_timing_enable(int on) {
+  timing_clear();
+  g_timing = on != 0;
+  return DFTB_OK;
+}
+
+extern "C" int dftb_gemm_timing_read(int cls, double* ms_total, long* launches, double* flops) {
+  if (cls < 0 || cls >= GT_NCLASS || !ms_total || !launches || !flops) return DFTB_ERR_ARG;
+  double ms = 0.0, fl = 0.0;
+  long n = 0;
+  for (auto& t : g_launches) {
+    if (t.cls != cls) continue;
+    if (hipEventSynchronize(t.b) != hipSuccess) return DFTB_ERR_LAUNCH;
+    float e = 0.0f;
+    if (hipEventElapsedTime(&e, t.a, t.b) != hipSuccess) return DFTB_ERR_LAUNCH;
+    ms += e;
+    fl += t.flops;
+    ++n;
+  }
+  *ms_total = ms;
+  *launches = n;
+  *flops = fl;
+  return DFTB_OK;
+}
+
 int gemm_launch(const GemmArgs& p0, hipStream_t st) {
   GemmArgs p = p0;
   if (p.M <= 0 || p.N <= 0 || p.batch <= 0) return DFTB_OK;
@@ -195,17 +261,21 @@ int gemm_launch(const GemmArgs& p0, hipStream_t st) {
   if (p.out_type == GEMM_OUT_BF16PAIR && !p.C2) return DFTB_ERR_ARG;
   const bool big = (p.M >= 128 && p.N >= 128) || p.tile == 128;
   const bool small = p.tile == 64 || !big;
+  if (p.in_type == GEMM_IN_BF16X3) return gemm_bf16x3_launch(p, st);
+  if (p.in_type != GEMM_IN_F64 && p.in_type != GEMM_IN_F32) return DFTB_ERR_ARG;
+  GemmTimer tm;
+  tm.begin(st);
   if (p.in_type == GEMM_IN_F64) {
     if (small) launch_fp<double, 64, 64, 16>(p, st);
     else launch_fp<double, 128, 128, 16>(p, st);
-  } else if (p.in_type == GEMM_IN_F32) {
+  } else {
     if (small) launch_fp<float, 64, 64, 32>(p, st);
     else launch_fp<float, 128, 128, 32>(p, st);
-  } else if (p.in_type == GEMM_IN_BF16X3) {
-    return gemm_bf16x3_launch(p, st);
-  } else {
-    return DFTB_ERR_ARG;
   }
+  const int cls = p.in_type == GEMM_IN_F64
+                      ? (small ? GT_F64_64 : ((p.a_trans || p.b_trans) ? GT_F64_128_T : GT_F64_128))
+                      : (small ? GT_F32_64 : GT_F32_128);
+  tm.end(st, cls, 2.0 * p.M * p.N * (double)p.K * p.batch);
   DFTB_CHECK_LAUNCH();
   return DFTB_OK;
 }

@@ -375,14 +375,23 @@ int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st) {
     const long span_a = ((long)P_BM * p.lda + p.K) * 2, span_b = ((long)P_BN * p.ldb + p.K) * 2;
     if (span_a >= 0xffff0000L || span_b >= 0xffff0000L) return DFTB_ERR_ARG;
     dim3 grid(ceil_div(p.N, P_BN), ceil_div(p.M, P_BM), p.batch * p.ksplit);
+    GemmTimer tm;
+    tm.begin(st);
     hipLaunchKernelGGL(gemm_bf3_pipe_kernel<DFTB_BF3_WAVES>, grid, dim3(DFTB_BF3_WAVES * 64), 0, st, p);
-  } else if (small) {
+    tm.end(st, GT_BF3_PIPE, 3.0 * 2.0 * p.M * p.N * (double)p.K * p.batch);
+    DFTB_CHECK_LAUNCH();
+    return DFTB_OK;
+  }
+  GemmTimer tm;
+  tm.begin(st);
+  if (small) {
     dim3 grid(ceil_div(p.N, 64), ceil_div(p.M, 64), p.batch * p.ksplit);
     hipLaunchKernelGGL((gemm_bf3_kernel<64, 64>), grid, dim3(256), 0, st, p);
   } else {
     dim3 grid(ceil_div(p.N, 128), ceil_div(p.M, 128), p.batch * p.ksplit);
     hipLaunchKernelGGL((gemm_bf3_kernel<128, 128>), grid, dim3(256), 0, st, p);
   }
+  tm.end(st, GT_BF3_TILE, 3.0 * 2.0 * p.M * p.N * (double)p.K * p.batch);
   DFTB_CHECK_LAUNCH();
   return DFTB_OK;
 }

@@ -118,6 +118,22 @@ def gram(X, P, Y, Q, n, out, ws, ldx=None, ldy=None, ldo=None, x_off=0, y_off=0,
     out.view(-1).as_strided((P, Q), (ldo, 1), o_off).copy_(red)
 
 
+def gemm_timing(on):
+    """Start (and clear) or stop the per-launch event timing of the GEMM kernels."""
+    call("dftb_gemm_timing_enable", int(bool(on)))
+
+
+GEMM_CLASSES = {"f64_128": 0, "f64_64": 1, "f32_128": 2, "f32_64": 3, "bf16x3_pipe": 4, "bf16x3_tile": 5,
+                "f64_128_t": 6}
+
+
+def gemm_timing_read(cls):
+    """(device seconds, launches, algorithmic flops) recorded for one kernel class."""
+    ms, n, fl = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()
+    call("dftb_gemm_timing_read", GEMM_CLASSES[cls], ctypes.byref(ms), ctypes.byref(n), ctypes.byref(fl))
+    return ms.value * 1e-3, int(n.value), fl.value
+
+
 class CasidaOperatorGPU:
     """Device-resident operator of the LR-TD-DFTB problem for one geometry.
 

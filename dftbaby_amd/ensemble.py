@@ -30,6 +30,7 @@ class EnsembleResult:
         self.XmY = []
         self.C = []
         self.iterations = []
+        self.vectors = []      # expansion vectors (one product pair each) per molecule
         self.resid = []
         self.t_build = 0.0
         self.t_solve = 0.0
@@ -125,13 +126,14 @@ def solve_operators(ops, nstates, conv=1.0e-6, maxiter=50, ifact=1, precision=0)
         if code != 1:
             raise ExcitedStatesError("ensemble member %d failed (code %d after %d iterations)" % (i, code, it))
         per[i] = (w.cpu().numpy(), op.from_padded(C).cpu().numpy(), op.from_padded(XmY).cpu().numpy(),
-                  op.from_padded(XpY).cpu().numpy(), it, resid.cpu().numpy())
-    for (w, C, XmY, XpY, it, r) in per:
+                  op.from_padded(XpY).cpu().numpy(), it, resid.cpu().numpy(), l)
+    for (w, C, XmY, XpY, it, r, l) in per:
         res.Omega.append(w)
         res.C.append(C)
         res.XmY.append(XmY)
         res.XpY.append(XpY)
         res.iterations.append(it)
+        res.vectors.append(l)
         res.resid.append(r)
     del keep
     return res

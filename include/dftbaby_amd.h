@@ -59,6 +59,20 @@ typedef struct GemmArgs {
 
 int dftb_gemm(const GemmArgs* p, void* stream);
 
+/* ---- launch timing of the GEMM kernels (measurement only) -----------------
+ * No reference counterpart.  dftb_gemm_timing_enable(1) clears the record and
+ * brackets every following GEMM launch with two HIP events on its stream;
+ * dftb_gemm_timing_read(cls, ...) waits for the recorded launches of kernel
+ * class cls and returns their summed device time (ms), count and algorithmic
+ * flops (2 M N K per batch entry, times 3 for the BF16x3 split product).
+ * Classes: 0 FP64 128x128 (no transposed operand), 1 FP64 64x64, 2 FP32
+ * 128x128, 3 FP32 64x64, 4 BF16x3 pipelined 256x128, 5 BF16x3 64/128 tiles,
+ * 6 FP64 128x128 with a transposed operand. */
+enum { GEMM_CLASS_F64_128 = 0, GEMM_CLASS_F64_64 = 1, GEMM_CLASS_F32_128 = 2, GEMM_CLASS_F32_64 = 3,
+       GEMM_CLASS_BF16X3_PIPE = 4, GEMM_CLASS_BF16X3_TILE = 5, GEMM_CLASS_F64_128_T = 6 };
+int dftb_gemm_timing_enable(int on);
+int dftb_gemm_timing_read(int cls, double* ms_total, long* launches, double* flops);
+
 /* ---- (a) Mulliken transition charges --------------------------------------
  * Replaces f2py tddftb.trans_charges(orbs_occ, orbs_virt, S, orbsPerAtom)
  * -> (q_oo, q_ov, q_vv)   (DFTB/extensions/tddftb.f90:12-112, called at
