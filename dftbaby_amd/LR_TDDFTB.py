@@ -40,6 +40,9 @@ class LR_TDDFTB:
     # from the model's initial space and preconditioner omega; plain Davidson
     # is the fallback when that run does not converge.
     guess_model = None
+    # long-range exchange products: "ao" (AO-basis GEMMs) or "charges" (the
+    # reference's gamma_lr-folded transition-charge contractions); same results
+    exchange = "ao"
 
     def __init__(self, atomlist=None, ground_state=None, **opts):
         if ground_state is None:
@@ -161,7 +164,7 @@ class LR_TDDFTB:
         if getattr(self.dftb2, "mulliken_dipoles", 0) == 1:
             raise NotImplementedError("multipole transition charges (mulliken_dipoles=1) are out of scope")
         self.op = CasidaOperatorGPU.from_orbitals(orbs[:, ao], orbs[:, av], S, self.dftb2.orbsPerAtom,
-                                                  self.omega, gamma, gamma_lr, lc)
+                                                  self.omega, gamma, gamma_lr, lc, exchange=self.exchange)
         torch.cuda.synchronize()
         self._t("transition charges", t0)
         self._Lambda2_calcOia()

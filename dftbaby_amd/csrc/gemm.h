@@ -23,6 +23,11 @@ __device__ __forceinline__ long gemm_out_offset(const GemmArgs& p, long row, lon
   return r + c;
 }
 
+// Hadamard factor of element (row, col), 1 when none is set.
+__device__ __forceinline__ double gemm_had(const GemmArgs& p, long row, long col) {
+  return p.had ? p.had[(row % p.hmod) * p.ldh + (col % p.hmod)] : 1.0;
+}
+
 static inline GemmArgs gemm_args_default() {
   GemmArgs p;
   p.A = p.A2 = p.B = p.B2 = nullptr;
@@ -35,6 +40,9 @@ static inline GemmArgs gemm_args_default() {
   p.rdiv = 0;
   p.rstr = 0;
   p.rs2 = 0;
+  p.had = nullptr;
+  p.hmod = 1;
+  p.ldh = 0;
   p.M = p.N = p.K = 0;
   p.batch = 1;
   p.ksplit = 1;

@@ -162,7 +162,8 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs p) {
         const int rr = IsF64<T>::value ? ((lane >> 4) + 4 * r) : (4 * (lane >> 4) + r);
         long row = m0 + wm * (BM / 2) + i * 16 + rr;
         long col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
-        if (row < M && col < N) store_out(p, cbase + remap_addr(p, row, col), p.alpha * (double)acc[i][j][r]);
+        if (row < M && col < N)
+          store_out(p, cbase + remap_addr(p, row, col), p.alpha * gemm_had(p, row, col) * (double)acc[i][j][r]);
       }
 }
 

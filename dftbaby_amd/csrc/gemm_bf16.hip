@@ -155,7 +155,8 @@ __global__ __launch_bounds__(256) void gemm_bf3_kernel(GemmArgs p) {
       for (int r = 0; r < 4; ++r) {
         long row = m0 + wm * (BM / 2) + i * 16 + 4 * (lane >> 4) + r;
         long col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
-        if (row < M && col < N) store_out_bf(p, cbase + gemm_out_offset(p, row, col), alpha * acc[i][j][r]);
+        if (row < M && col < N)
+          store_out_bf(p, cbase + gemm_out_offset(p, row, col), alpha * (float)gemm_had(p, row, col) * acc[i][j][r]);
       }
 }
 
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
     const long col = n0 + wn * 64 + j * 16 + (lane & 15);                      \
     if (row < M && col < N) {                                                  \
       const long off = addr(row, col);                                         \
-      const float val = alpha * acc[i][j][r];                                  \
+      const float val = alpha * (float)gemm_had(p, row, col) * acc[i][j][r];   \
       STMT;                                                                    \
     }                                                                          \
   }

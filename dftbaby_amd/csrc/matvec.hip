@@ -23,8 +23,20 @@ struct MvLayout {
   long N;        // vector length opad*vpad
   int ks;        // split-K slices for the Coulomb projection
   int ac;        // atoms per exchange slice
-  size_t vin, tpart, t, g, g32, x1, x2, tu, total;
+  int aoc;       // vectors per AO-exchange chunk
+  size_t vin, tpart, t, g, g32, x1, x2, tu, ao, total;
 };
+
+// AO exchange intermediates per vector (elements): B1 [2 npad][max(opad, vpad)],
+// BH [2 npad][2 npad], B3 [vpad][2 npad]
+static void ao_sizes(const CasidaOperator* op, long* e1, long* eh, long* e3) {
+  const long np2 = 2L * op->npad, w = op->opad > op->vpad ? op->opad : op->vpad;
+  *e1 = np2 * w;
+  *eh = np2 * np2;
+  *e3 = (long)op->vpad * np2;
+}
+// bytes per element summed over the arrays of one operand (BF16x3: hi + lo)
+static int ao_esz(int precision) { return precision == 0 ? 8 : 4; }
 
 static int in_bytes(int precision) { return precision == 0 ? 8 : 4; }  // f64, f32, bf16 hi+lo
 
@@ -56,9 +68,23 @@ static MvLayout mv_layout(const CasidaOperator* op, int nvec, bool hermitian) {
   L.g = off; off += al256((size_t)nvec * op->nat * 8);
   L.g32 = off; off += al256((size_t)nvec * op->nat * 4);
   bool exch = !hermitian && op->lc;
+  const bool ao = exch && op->exchange == 1;
   L.x1 = off; off += exch ? al256((size_t)nvec * L.N * 8) : 0;
   L.x2 = off; off += exch ? al256((size_t)nvec * L.N * 8) : 0;
-  L.tu = off; off += exch ? al256(per_atom * ac) : 0;
+  L.tu = off; off += (exch && !ao) ? al256(per_atom * ac) : 0;
+  L.aoc = 0;
+  L.ao = off;
+  if (ao) {
+    long e1, eh, e3;
+    ao_sizes(op, &e1, &eh, &e3);
+    const size_t per_vec = (size_t)(e1 + eh + e3) * ao_esz(op->precision);
+    const size_t budget = (size_t)8 << 30;
+    long c = (long)(budget / (per_vec ? per_vec : 1));
+    if (c < 1) c = 1;
+    if (c > nvec) c = nvec;
+    L.aoc = (int)c;
+    off += al256(per_vec * c + 3 * 256);
+  }
   L.total = off;
   return L;
 }
@@ -213,13 +239,95 @@ static int coulomb(const CasidaOperator* op, const MvLayout& L, char* ws, const 
   return gemm_launch(c, st);
 }
 
+// One GEMM of the AO exchange in the operand type of the precision: C = alpha A B^T
+// (btrans = 1) or alpha A B (btrans = 0, FP64/FP32 only); final = FP64 output.
+static int ao_gemm(int prec, const void* A, const void* A2, long lda, const void* B, const void* B2, long ldb,
+                   long sB, int btrans, void* C, void* C2, long ldc, long sC, int M, int N, int K, int batch,
+                   double alpha, bool final, const CasidaOperator* op, bool had, hipStream_t st) {
+  GemmArgs g = gemm_args_default();
+  g.in_type = prec == 0 ? GEMM_IN_F64 : (prec == 1 ? GEMM_IN_F32 : GEMM_IN_BF16X3);
+  g.out_type = final ? GEMM_OUT_F64 : (prec == 0 ? GEMM_OUT_F64 : (prec == 1 ? GEMM_OUT_F32 : GEMM_OUT_BF16PAIR));
+  g.A = A; g.A2 = A2; g.lda = lda; g.sA = 0;
+  g.B = B; g.B2 = B2; g.ldb = ldb; g.sB = sB; g.b_trans = btrans;
+  g.C = C; g.C2 = C2; g.ldc = ldc; g.sC = sC;
+  g.M = M; g.N = N; g.K = K; g.batch = batch; g.alpha = alpha;
+  if (had) { g.had = op->gam_ao; g.hmod = op->npad; g.ldh = op->npad; }
+  return gemm_launch(g, st);
+}
+
+// Long-range exchange in the AO basis.  With the Mulliken transition charges
+// q^{pq}_A = 1/2 sum_{mu in A} (C_mu,p Y_mu,q + Y_mu,p C_mu,q), Y = S C, the two
+// exchange products of a vector V (o x v) are
+//   K1 V = 1/4 [C_o; Y_o]^T { G o ([Y_o; C_o] V [Y_v; C_v]^T) } [C_v; Y_v]
+//   K2 V = 1/4 [C_o; Y_o]^T { G o ([Y_v; C_v] V^T [Y_o; C_o]^T) } [C_v; Y_v]
+// (G: gamma_lr expanded to orbital pairs, o: Hadamard product), i.e. four GEMMs
+// each over 2 npad-sized AO dimensions instead of the Nat x o x v x (o + v)
+// transition-charge contractions.  Every GEMM is in the K-contiguous (NT) form.
+static int ao_exchange(const CasidaOperator* op, const MvLayout& L, char* ws, int nvec, const void* V,
+                       const void* Vlo, const void* VT, const void* VTlo, double* X1, double* X2, hipStream_t st) {
+  const int prec = op->precision;
+  const long np2 = 2L * op->npad, opad = op->opad, vpad = op->vpad, N = L.N;
+  long e1, eh, e3;
+  ao_sizes(op, &e1, &eh, &e3);
+  const long vs = prec == 0 ? 8 : (prec == 1 ? 4 : 2);   // bytes per vector element (per array)
+  const long es = vs;                                     // intermediates use the same element type
+  int rc;
+  for (int n0 = 0; n0 < nvec; n0 += L.aoc) {
+    const int c = (nvec - n0 < L.aoc) ? (nvec - n0) : L.aoc;
+    char* b1 = (char*)(((uintptr_t)(ws + L.ao) + 255) & ~(uintptr_t)255);
+    char* bh = b1 + (size_t)c * e1 * es;
+    char* b3 = bh + (size_t)c * eh * es;
+    char *b1l = nullptr, *bhl = nullptr, *b3l = nullptr;
+    if (prec == 2) {
+      b1l = b3 + (size_t)c * e3 * es;
+      bhl = b1l + (size_t)c * e1 * es;
+      b3l = bhl + (size_t)c * eh * es;
+    }
+    const char* v0 = (const char*)V + (size_t)n0 * N * vs;
+    const char* v0l = Vlo ? (const char*)Vlo + (size_t)n0 * N * vs : nullptr;
+    const char* t0 = VT ? (const char*)VT + (size_t)n0 * N * vs : nullptr;
+    const char* t0l = VTlo ? (const char*)VTlo + (size_t)n0 * N * vs : nullptr;
+    double* x1 = X1 + (size_t)n0 * N;
+    double* x2 = X2 + (size_t)n0 * N;
+    // ---- K1: B1 = [Y_v; C_v] V^T, BH = G o ([Y_o; C_o] B1^T), B3 = [C_v; Y_v]^T BH^T, X1 = 1/4 W B3^T
+    if ((rc = ao_gemm(prec, op->ao_v[0], op->ao_v[1], vpad, v0, v0l, vpad, N, 1, b1, b1l, opad, e1, (int)np2,
+                      (int)opad, (int)vpad, c, 1.0, false, op, false, st))) return rc;
+    if ((rc = ao_gemm(prec, op->ao_o[0], op->ao_o[1], opad, b1, b1l, opad, e1, 1, bh, bhl, np2, eh, (int)np2,
+                      (int)np2, (int)opad, c, 1.0, false, op, true, st))) return rc;
+    if ((rc = ao_gemm(prec, op->ao_cyvt[0], op->ao_cyvt[1], np2, bh, bhl, np2, eh, 1, b3, b3l, np2, e3,
+                      (int)vpad, (int)np2, (int)np2, c, 1.0, false, op, false, st))) return rc;
+    if ((rc = ao_gemm(prec, op->ao_w[0], op->ao_w[1], np2, b3, b3l, np2, e3, 1, x1, nullptr, vpad, N, (int)opad,
+                      (int)vpad, (int)np2, c, 0.25, true, op, false, st))) return rc;
+    // ---- K2: B1 = [Y_o; C_o] V, BH = G o ([Y_v; C_v] B1^T), B3, X2 as above
+    if (prec == 2) {
+      if ((rc = ao_gemm(prec, op->ao_o[0], op->ao_o[1], opad, t0, t0l, opad, N, 1, b1, b1l, vpad, e1, (int)np2,
+                        (int)vpad, (int)opad, c, 1.0, false, op, false, st))) return rc;
+    } else {
+      if ((rc = ao_gemm(prec, op->ao_o[0], nullptr, opad, v0, nullptr, vpad, N, 0, b1, nullptr, vpad, e1,
+                        (int)np2, (int)vpad, (int)opad, c, 1.0, false, op, false, st))) return rc;
+    }
+    if ((rc = ao_gemm(prec, op->ao_v[0], op->ao_v[1], vpad, b1, b1l, vpad, e1, 1, bh, bhl, np2, eh, (int)np2,
+                      (int)np2, (int)vpad, c, 1.0, false, op, true, st))) return rc;
+    if ((rc = ao_gemm(prec, op->ao_cyvt[0], op->ao_cyvt[1], np2, bh, bhl, np2, eh, 1, b3, b3l, np2, e3,
+                      (int)vpad, (int)np2, (int)np2, c, 1.0, false, op, false, st))) return rc;
+    if ((rc = ao_gemm(prec, op->ao_w[0], op->ao_w[1], np2, b3, b3l, np2, e3, 1, x2, nullptr, vpad, N, (int)opad,
+                      (int)vpad, (int)np2, c, 0.25, true, op, false, st))) return rc;
+  }
+  return DFTB_OK;
+}
+
 static int check_op(const CasidaOperator* op) {
   if (!op) return DFTB_ERR_ARG;
   if (op->nat <= 0 || op->nocc <= 0 || op->nvirt <= 0) return DFTB_ERR_ARG;
   if (op->opad < op->nocc || op->vpad < op->nvirt || (op->opad & 7) || (op->vpad & 7)) return DFTB_ERR_ARG;
   if (op->precision < 0 || op->precision > 2) return DFTB_ERR_ARG;
   if (!op->omega || !op->gamma || !op->q_c) return DFTB_ERR_ARG;
-  if (op->lc) {
+  if (op->lc && op->exchange == 1) {
+    if (op->npad <= 0 || (op->npad & 7) || !op->gam_ao) return DFTB_ERR_ARG;
+    if (!op->ao_v[0] || !op->ao_o[0] || !op->ao_cyvt[0] || !op->ao_w[0]) return DFTB_ERR_ARG;
+    if (op->precision == 2 && (!op->ao_v[1] || !op->ao_o[1] || !op->ao_cyvt[1] || !op->ao_w[1])) return DFTB_ERR_ARG;
+  } else if (op->lc) {
+    if (op->exchange != 0) return DFTB_ERR_ARG;
     if (!op->q_oo[0] || !op->q_ov[0] || !op->qf_ov[0] || !op->qf_vv[0]) return DFTB_ERR_ARG;
     if (op->precision == 2 && (!op->q_oo[1] || !op->q_ov[1] || !op->qf_ov[1] || !op->qf_vv[1])) return DFTB_ERR_ARG;
   }
@@ -280,7 +388,13 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
     void* T = tu;
     void* Tlo = prec == 2 ? (void*)(tu + tu_elems * 2) : nullptr;
     const long esz = prec == 0 ? 8 : (prec == 1 ? 4 : 2);
-    const int nat_eff = (op->atom_limit > 0 && op->atom_limit < op->nat) ? op->atom_limit : op->nat;
+    const int nat_eff = (op->exchange == 1) ? 0
+                        : ((op->atom_limit > 0 && op->atom_limit < op->nat) ? op->atom_limit : op->nat);
+    if (op->exchange == 1) {
+      const unsigned short* VTh_ = prec == 2 ? (const unsigned short*)Vlo + total : nullptr;
+      const unsigned short* VTl_ = prec == 2 ? VTh_ + total : nullptr;
+      if ((rc = ao_exchange(op, L, ws, nvec, Vx, Vlo, VTh_, VTl_, X1, X2, st))) return rc;
+    }
     for (int A0 = 0; prec < 2 && A0 < nat_eff; A0 += L.ac) {
       const int ac = (nat_eff - A0 < L.ac) ? (nat_eff - A0) : L.ac;
       // ---- K1 expand: T[n][a'][j][a] = sum_b V[n][j][b] qf_vv[b][A0+a'][a]

@@ -150,48 +150,97 @@ class CasidaOperatorGPU:
         self.lc = int(lc)
         dev = self.device
         om = torch.ones((self.opad, self.vpad), dtype=torch.float64, device=dev)
-        om[:nocc, :nvirt] = torch.as_tensor(np.asarray(omega, dtype=np.float64), device=dev)
+        om[:nocc, :nvirt] = (omega.to(dev, torch.float64) if torch.is_tensor(omega)
+                             else torch.as_tensor(np.asarray(omega, dtype=np.float64), device=dev))
         self.omega = om.reshape(-1).contiguous()
         self.omega_real = torch.zeros_like(om)
         self.omega_real[:nocc, :nvirt] = om[:nocc, :nvirt]
-        self.gamma = torch.as_tensor(np.asarray(gamma, dtype=np.float64), device=dev).contiguous()
-        self.gamma_lr = torch.as_tensor(np.asarray(gamma_lr, dtype=np.float64), device=dev).contiguous()
+        g = lambda a: (a.to(dev, torch.float64).contiguous() if torch.is_tensor(a)
+                       else torch.as_tensor(np.asarray(a, dtype=np.float64), device=dev).contiguous())
+        self.gamma = g(gamma)
+        self.gamma_lr = g(gamma_lr)
         self.ws = Workspace(dev)
         self.q = {}          # FP64 engine-layout charges
         self.qx = {}         # low-precision operand copies keyed by precision
         self.qii = self.qaa = None
+        self.exchange = "charges"
+        self.ao = None
+        self.npad = 0
+        self.orbs = None
         self.tuning = {}     # precision -> {"atom_chunk", "tile_expand", "tile_contract"} (autotune.py)
 
     # -- construction ---------------------------------------------------------
     @classmethod
-    def from_orbitals(cls, orbs_occ, orbs_virt, S, orbsPerAtom, omega, gamma, gamma_lr, lc, device=None):
-        """Transition charges on the GPU (kernel (a)) straight into the engine layouts."""
+    def from_orbitals(cls, orbs_occ, orbs_virt, S, orbsPerAtom, omega, gamma, gamma_lr, lc, device=None,
+                      exchange="ao"):
+        """Transition charges on the GPU (kernel (a)) straight into the engine layouts.
+
+        exchange="ao": the long-range exchange products run in the AO basis
+        (no q_oo / q_vv / folded charges are stored); "charges": through the
+        gamma_lr-folded transition charges (the reference's contraction)."""
+        if exchange not in ("ao", "charges"):
+            raise ValueError("exchange must be 'ao' or 'charges'")
         opa = np.ascontiguousarray(np.asarray(orbsPerAtom, dtype=np.int32))
         nat = len(opa)
         self = cls(nat, orbs_occ.shape[1], orbs_virt.shape[1], omega, gamma, gamma_lr, lc, device)
         dev = self.device
-        t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=dev)
+        t = lambda a: (a.to(dev, torch.float64).contiguous() if torch.is_tensor(a)
+                       else torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=dev))
         Co, Cv, St = t(orbs_occ), t(orbs_virt), t(S)
         norb = St.shape[0]
         o, v, op, vp = self.nocc, self.nvirt, self.opad, self.vpad
+        self.exchange = exchange if self.lc else "charges"
+        self.orbs = (Co, Cv, St, opa)
         z = lambda *s: torch.empty(s, dtype=torch.float64, device=dev)
+        ao = self.exchange == "ao"
         q_c = z(nat, op, vp)
-        q_oo = z(o, nat, op)           # also needed (diagonal only) for Lambda2 when lc = 0
-        q_ov = z(o, nat, vp) if self.lc else None
-        q_vv = z(vp, nat, vp)
+        q_oo = None if ao else z(o, nat, op)
+        q_ov = z(o, nat, vp) if (self.lc and not ao) else None
+        q_vv = None if ao else z(vp, nat, vp)
         nb = _lib.load().dftb_trans_charges_workspace(norb, o, v, nat)
         w, wn = self.ws.get(nb)
         call("dftb_trans_charges_engine", _p(Co), _p(Cv), _p(St),
              opa.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), nat, norb, o, v,
              _p(q_oo), _p(q_ov), _p(q_vv), _p(q_c), _p(w), wn, _stream())
         self.q["c"] = q_c
-        self._diagonals_from_engine(q_c, q_vv)
-        self.qii = q_oo[:, :, :o].diagonal(dim1=0, dim2=2).T.contiguous()   # q^{ii}_A as [o][nat]
-        if self.lc:
-            self.q["oo"], self.q["ov"] = q_oo, q_ov
-            self._fold(q_vv)
+        if ao:
+            self._build_ao(Co, Cv, St, opa)
+        else:
+            self._diagonals_from_engine(q_c, q_vv)
+            self.qii = q_oo[:, :, :o].diagonal(dim1=0, dim2=2).T.contiguous()   # q^{ii}_A as [o][nat]
+            if self.lc:
+                self.q["oo"], self.q["ov"] = q_oo, q_ov
+                self._fold(q_vv)
         del q_vv, q_oo
         return self
+
+    def _build_ao(self, Co, Cv, St, opa):
+        """AO-basis exchange operands ([Y_v; C_v], [Y_o; C_o], [C_v; Y_v]^T, [C_o; Y_o]^T with Y = S C,
+        gamma_lr expanded to orbital pairs) and the diagonal charges q^{ii}_A, q^{aa}_A."""
+        dev = self.device
+        norb = St.shape[0]
+        o, v, op, vp, nat = self.nocc, self.nvirt, self.opad, self.vpad, self.nat
+        npad = pad8(norb)
+        Yo = torch.empty((norb, o), dtype=torch.float64, device=dev)
+        Yv = torch.empty((norb, v), dtype=torch.float64, device=dev)
+        gemm(St, Co, Yo, norb, o, norb, norb, o, o)
+        gemm(St, Cv, Yv, norb, v, norb, norb, v, v)
+        zz = lambda *s: torch.zeros(s, dtype=torch.float64, device=dev)
+        ao_v, ao_o = zz(2 * npad, vp), zz(2 * npad, op)
+        ao_v[:norb, :v], ao_v[npad:npad + norb, :v] = Yv, Cv
+        ao_o[:norb, :o], ao_o[npad:npad + norb, :o] = Yo, Co
+        ao_cyvt, ao_w = zz(vp, 2 * npad), zz(op, 2 * npad)
+        ao_cyvt[:v, :norb], ao_cyvt[:v, npad:npad + norb] = Cv.T, Yv.T
+        ao_w[:o, :norb], ao_w[:o, npad:npad + norb] = Co.T, Yo.T
+        amap = torch.as_tensor(np.repeat(np.arange(nat), opa), device=dev)
+        gam = zz(npad, npad)
+        if self.lc:
+            gam[:norb, :norb] = self.gamma_lr[amap][:, amap]
+        self.npad = npad
+        self.ao = {"v": ao_v, "o": ao_o, "cyvt": ao_cyvt, "w": ao_w, "gam": gam.contiguous()}
+        # q^{ii}_A = sum_{mu in A} C_mu,i Y_mu,i ; q^{aa}_A likewise  ->  [o][nat], [v][nat]
+        self.qii = zz(nat, o).index_add_(0, amap, Co * Yo).T.contiguous()
+        self.qaa = zz(nat, v).index_add_(0, amap, Cv * Yv).T.contiguous()
 
     @classmethod
     def from_charges(cls, q_oo, q_vv, q_ov, omega, gamma, gamma_lr, lc, device=None):
@@ -253,7 +302,19 @@ class CasidaOperatorGPU:
         qc32 = torch.empty(self.q["c"].shape, dtype=torch.float32, device=dev)
         call("dftb_convert_f32", _p(self.q["c"]), self.q["c"].numel(), _p(qc32), _stream())
         cp["c"] = qc32
-        if self.lc:
+        if self.lc and self.exchange == "ao":
+            for key in ("v", "o", "cyvt", "w"):
+                x = self.ao[key]
+                if precision == 1:
+                    y = torch.empty(x.shape, dtype=torch.float32, device=dev)
+                    call("dftb_convert_f32", _p(x), x.numel(), _p(y), _stream())
+                    cp["ao_" + key] = (y, None)
+                else:
+                    hi = torch.empty(x.shape, dtype=torch.int16, device=dev)
+                    lo = torch.empty(x.shape, dtype=torch.int16, device=dev)
+                    call("dftb_convert_bf16pair", _p(x), x.numel(), _p(hi), _p(lo), _stream())
+                    cp["ao_" + key] = (hi, lo)
+        elif self.lc:
             if precision == 1:
                 for key in ("oo", "ov", "qf_ov", "qf_vv"):
                     x = self.q[key]
@@ -290,6 +351,22 @@ class CasidaOperatorGPU:
         s.tile_expand, s.tile_contract, s.atom_limit = te, tc, atom_limit
         s.omega = self.omega.data_ptr()
         s.gamma = self.gamma.data_ptr()
+        if self.lc and self.exchange == "ao":
+            s.exchange, s.npad = 1, self.npad
+            s.gam_ao = self.ao["gam"].data_ptr()
+            if precision == 0:
+                s.q_c = self.q["c"].data_ptr()
+                for key in ("v", "o", "cyvt", "w"):
+                    getattr(s, "ao_" + key)[0] = self.ao[key].data_ptr()
+            else:
+                self.prepare(precision)
+                cp = self.qx[precision]
+                s.q_c = cp["c"].data_ptr()
+                for key in ("v", "o", "cyvt", "w"):
+                    hi, lo = cp["ao_" + key]
+                    getattr(s, "ao_" + key)[0] = hi.data_ptr()
+                    getattr(s, "ao_" + key)[1] = lo.data_ptr() if lo is not None else 0
+            return s
         if precision == 0:
             s.q_c = self.q["c"].data_ptr()
             if self.lc:

@@ -51,7 +51,7 @@ def build_operators(ground_states, nr_active_occ=None, nr_active_virt=None):
         g = gs.getGamma()
         glr = gs.getGamma_lr() if lc else 0.0 * g
         ops.append(CasidaOperatorGPU.from_orbitals(C[:, ao], C[:, av], gs.getOverlapMatrix(), gs.orbsPerAtom, omega,
-                                                   g, glr, lc))
+                                                   g, glr, lc, exchange="charges"))
     return ops
 
 

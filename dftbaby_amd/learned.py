@@ -302,7 +302,24 @@ def learned_start(op, model, nstates, M=None, use_shift=False, rank="model"):
     gemm(qsel, X, Kc, M, M, op.nat, M, M, M, a_trans=1)
     ApB = torch.empty((M, M), dtype=torch.float64, device=dev)
     AmB = torch.empty((M, M), dtype=torch.float64, device=dev)
-    if op.lc:
+    if op.lc and op.exchange == "ao":
+        # the operator keeps no transition-charge layouts: build them for the
+        # occupied / virtual orbitals the selected pairs touch
+        from .engine import CasidaOperatorGPU
+        ii, aa = sel // op.vpad, sel % op.vpad
+        isel, ilocal = np.unique(ii, return_inverse=True)
+        asel, alocal = np.unique(aa, return_inverse=True)
+        Co, Cv, St, opa = op.orbs
+        om = op.omega_real[torch.as_tensor(isel, device=dev)][:, torch.as_tensor(asel, device=dev)]
+        sub = CasidaOperatorGPU.from_orbitals(Co[:, torch.as_tensor(isel, device=dev)].contiguous(),
+                                              Cv[:, torch.as_tensor(asel, device=dev)].contiguous(), St, opa,
+                                              om, op.gamma, op.gamma_lr, 1, dev, exchange="charges")
+        ipl = torch.as_tensor(ilocal.astype(np.int32), device=dev)
+        apl = torch.as_tensor(alocal.astype(np.int32), device=dev)
+        call("dftb_restricted_ab", _p(ipl), _p(apl), M, sub.nat, sub.opad, sub.vpad, _p(sub.omega), _p(Kc),
+             _p(sub.q["oo"]), _p(sub.q["ov"]), _p(sub.q["qf_ov"]), _p(sub.q["qf_vv"]), 1, _p(ApB), _p(AmB), st)
+        del sub
+    elif op.lc:
         call("dftb_restricted_ab", _p(ip), _p(ap), M, op.nat, op.opad, op.vpad, _p(op.omega), _p(Kc),
              _p(op.q["oo"]), _p(op.q["ov"]), _p(op.q["qf_ov"]), _p(op.q["qf_vv"]), 1, _p(ApB), _p(AmB), st)
     else:

@@ -274,7 +274,7 @@ def test_trans_charges_engine_layout():
     q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
     omega, _ = O.orbital_differences(orbe, f, ao, av)
     op = CasidaOperatorGPU.from_orbitals(C[:, ao], C[:, av], S, gs.orbsPerAtom, omega, gs.getGamma(),
-                                         gs.getGamma_lr(), 1)
+                                         gs.getGamma_lr(), 1, exchange="charges")
     nat, o, v = q_ov.shape
     qc = op.q["c"].cpu().numpy()
     np.testing.assert_allclose(qc[:, :o, :v], q_ov, atol=1e-13)
@@ -290,7 +290,7 @@ def test_trans_charges_engine_layout():
 # ---------------------------------------------------------------------------
 # (b) matvecs
 # ---------------------------------------------------------------------------
-def _operator(name, lc, sigma=0.05, seed=3):
+def _operator(name, lc, sigma=0.05, seed=3, exchange="ao"):
     from dftbaby_amd.engine import CasidaOperatorGPU
     gs = ground_state(name, lc, sigma, seed)
     f, orbe, C, S = gs.getOccupation(), gs.getKSEnergies(), gs.getKSCoefficients(), gs.getOverlapMatrix()
@@ -298,13 +298,15 @@ def _operator(name, lc, sigma=0.05, seed=3):
     q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
     omega, _ = O.orbital_differences(orbe, f, ao, av)
     g, glr = gs.getGamma(), gs.getGamma_lr()
-    op = CasidaOperatorGPU.from_orbitals(C[:, ao], C[:, av], S, gs.orbsPerAtom, omega, g, glr, lc)
+    op = CasidaOperatorGPU.from_orbitals(C[:, ao], C[:, av], S, gs.orbsPerAtom, omega, g, glr, lc, exchange=exchange)
     return op, (g, glr, q_oo, q_ov, q_vv, omega)
 
 
+@pytest.mark.parametrize("exchange", ["ao", "charges"])
 @pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 3e-5), (2, 3e-5)])
-def test_casida_matvec(precision, tol, rng):
-    op, (g, glr, q_oo, q_ov, q_vv, omega) = _operator("C12H14", 1)
+def test_casida_matvec(exchange, precision, tol, rng):
+    """Both exchange formulations (AO-basis GEMMs, transition-charge contractions) vs the oracle."""
+    op, (g, glr, q_oo, q_ov, q_vv, omega) = _operator("C12H14", 1, exchange=exchange)
     o, v = omega.shape
     nvec = 5
     vs = rng.standard_normal((o, v, nvec))
@@ -322,11 +324,13 @@ def test_casida_matvec(precision, tol, rng):
     assert np.all(Pp[:, o:, :] == 0) and np.all(Pp[:, :, v:] == 0)
 
 
-@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (2, 3e-5)])
-def test_casida_matvec_flake(precision, tol, rng):
+@pytest.mark.parametrize("exchange,precision,tol", [("charges", 0, 1e-12), ("charges", 2, 3e-5), ("ao", 0, 1e-12),
+                                                    ("ao", 1, 3e-5), ("ao", 2, 3e-5)])
+def test_casida_matvec_flake(exchange, precision, tol, rng):
     """Armchair flake C180H36 (o = v = 378): the exchange GEMMs take the large-tile
-    pipelined path (expands with row remap, contracts with M >= 256)."""
-    op, (g, glr, q_oo, q_ov, q_vv, omega) = _operator("C180H36", 1, 0.05, 5)
+    pipelined path (charges: expands with row remap, contracts with M >= 256;
+    AO: 2 Norb-sized GEMMs with the gamma_lr Hadamard epilogue)."""
+    op, (g, glr, q_oo, q_ov, q_vv, omega) = _operator("C180H36", 1, 0.05, 5, exchange=exchange)
     o, v = omega.shape
     assert o >= 256 and v >= 256
     nvec = 3
@@ -341,7 +345,7 @@ def test_casida_matvec_flake(precision, tol, rng):
 
 def test_casida_matvec_atom_chunks(rng):
     """Exchange contractions split over atom slices accumulate to the same product."""
-    op, (g, glr, q_oo, q_ov, q_vv, omega) = _operator("C8H10", 1)
+    op, (g, glr, q_oo, q_ov, q_vv, omega) = _operator("C8H10", 1, exchange="charges")
     o, v = omega.shape
     V = op.to_padded(rng.standard_normal((3, o, v)))
     P1, M1 = op.matvec(V, 0, atom_chunk=0)
