@@ -462,6 +462,8 @@ class DavidsonStats:
         self.matvec_vectors = 0
         self.t_total = 0.0
         self.learned = False
+        self.refined = False
+        self.l = 0
         self._events = []
 
     def event_pair(self):
@@ -520,13 +522,17 @@ def _orthonormalize_block(bs, l, m, N, ws, st):
 
 def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, XmYguess=None, XpYguess=None,
                               w_guess=None, precision=0, verbose=0, stats=None, initial_space=None,
-                              precond_omega=None):
+                              precond_omega=None, refine=True):
     """Non-Hermitian Davidson on the GPU (Solver.py:637-917).
 
     Returns padded device tensors (Omega [k], C, XmY, XpY [k][N]) and stats.
+    With precision > 0 and refine, a run whose largest residual stops falling
+    (the noise floor of the low-precision products) restarts from its current
+    X-Y, X+Y with FP64 products, which converge to conv in a few iterations.
     """
     dev = op.device
     stats = stats or DavidsonStats()
+    it0 = stats.iterations
     t_start = time.time()
     k = int(nstates)
     o, v, N = op.nocc, op.nvirt, op.N
@@ -575,6 +581,8 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
     norms = torch.empty(2 * k, dtype=torch.float64, device=dev)
     keep = torch.empty(2 * k, dtype=torch.int32, device=dev)
     w2 = None
+    refine_now = False
+    best_res, stall = np.inf, 0
     for it in range(maxiter):
         if XpYguess is None or it > 0:
             m = l - new0
@@ -637,9 +645,20 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
             for n in range(k):
                 print("  Eigenvalue %3d      %2.7f    %s   res. norm = %e" %
                       (n + 1, w_host[n], "converged    " if nsum[n] < conv else "not converged", nsum[n]))
-        stats.iterations = it + 1
+        stats.iterations = it0 + it + 1
         if (nsum < conv).all() and it > 0:
             break
+        if precision > 0 and refine:
+            # the low-precision noise floor only matters close to convergence: stagnation is
+            # counted once the largest residual is within 100 x conv (early plateaus are normal)
+            r = float(nsum.max())
+            if r < 0.7 * best_res or r > max(100.0 * conv, 1e-4):
+                best_res, stall = min(best_res, r), 0
+            else:
+                stall += 1
+            if stall >= 3 or it == maxiter - 1:
+                refine_now = True
+                break
         sel = _precondition_select(nL, nR, k, l, kmax, conv)
         dk = len(sel)
         if dk == 0:
@@ -666,6 +685,14 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
         raise ExcitedStatesNotConverged(
             "Davidson-like diagonalization for non-Hermitian eigenvalue problem did not converge after %s "
             "iterations (threshold: %e) !!!" % (maxiter, conv))
+    if refine_now:
+        V0 = torch.cat([L, R], 0).contiguous()
+        stats.refined = True
+        out = nonhermitian_davidson_gpu(op, k, conv=conv, maxiter=max(maxiter - (it + 1), 10), precision=0,
+                                        verbose=verbose, stats=stats, initial_space=V0, precond_omega=precond_omega,
+                                        refine=False)
+        stats.t_total = time.time() - t_start
+        return out
     # C = bs Tb (Solver.py:906-910)
     C = torch.empty((k, N), dtype=torch.float64, device=dev)
     gemm(Tb, bs, C, k, N, l, k, N, N, a_trans=1, tile=64)
