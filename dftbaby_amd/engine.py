@@ -107,7 +107,10 @@ def gram(X, P, Y, Q, n, out, ws, ldx=None, ldy=None, ldo=None, x_off=0, y_off=0,
     ldo = ldo or Q
     if P == 0 or Q == 0:
         return
-    ks = max(1, min(64, (n + 65535) // 65536))
+    # enough K slices to fill the GPU (>= ~2048 workgroups of 64 x 64 tiles) with chunks of
+    # >= 2048 elements, so the small operand's chunk stays in L2 across the row tiles
+    tiles = ((P + 63) // 64) * ((Q + 63) // 64)
+    ks = max(1, min(512, (n + 2047) // 2048, (2048 + tiles - 1) // tiles))
     kchunk = ((n + ks - 1) // ks + 31) // 32 * 32
     ks = (n + kchunk - 1) // kchunk
     part, _ = ws.get(ks * P * Q * 8)
@@ -463,6 +466,7 @@ class DavidsonStats:
         self.t_total = 0.0
         self.learned = False
         self.refined = False
+        self.restarts = 0
         self.l = 0
         self._events = []
 
@@ -520,15 +524,46 @@ def _orthonormalize_block(bs, l, m, N, ws, st):
     return l + len(kept)
 
 
+def _collapse(sub, Rb, Lb, l, k, N, ws, st):
+    """Thick restart: replace the basis by the orthonormalised span of the current
+    Ritz coordinates [Rb, Lb] (X+Y and X-Y); basis vectors, their stored products
+    and the projected matrices follow by the same linear combination (no matvec)."""
+    dev = sub.bs.device
+    X = torch.cat([Rb[:l, :k].T, Lb[:l, :k].T], 0).contiguous()       # rows: subspace coordinates
+    keep = torch.empty(2 * k, dtype=torch.int32, device=dev)
+    nb = _lib.load().dftb_orthonormalize_workspace(0, 2 * k, l)
+    wsb, wsn = ws.get(nb)
+    call("dftb_orthonormalize", None, 0, _p(X), 2 * k, l, 1.0e-8, _p(keep), _p(wsb), wsn, st)
+    kept = torch.as_tensor(np.where(keep.cpu().numpy() != 0)[0], device=dev)
+    Q = X[kept].contiguous()                                          # [r][l], orthonormal rows
+    r = Q.shape[0]
+    tmp = torch.empty((r, N), dtype=torch.float64, device=dev)
+    for B in [sub.bs] + sub.prod:
+        gemm(Q, B, tmp, r, N, l, l, N, N)
+        B[:r].copy_(tmp)
+    T1 = torch.empty((r, l), dtype=torch.float64, device=dev)
+    Mn = torch.empty((r, r), dtype=torch.float64, device=dev)
+    for Mx in sub.M:
+        gemm(Q, Mx, T1, r, l, l, l, sub.cap, l)
+        gemm(T1, Q, Mn, r, r, l, l, l, r, b_trans=1)
+        Mx[:r, :r].copy_(Mn)
+    return r
+
+
 def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, XmYguess=None, XpYguess=None,
                               w_guess=None, precision=0, verbose=0, stats=None, initial_space=None,
-                              precond_omega=None, refine=True):
+                              precond_omega=None, refine=True, restart=None):
     """Non-Hermitian Davidson on the GPU (Solver.py:637-917).
 
     Returns padded device tensors (Omega [k], C, XmY, XpY [k][N]) and stats.
     With precision > 0 and refine, a run whose largest residual stops falling
     (the noise floor of the low-precision products) restarts from its current
     X-Y, X+Y with FP64 products, which converge to conv in a few iterations.
+    restart (subspace size; None or 0 = never, as the reference): before the
+    basis would exceed it, the basis collapses to span(X+Y, X-Y) of the current
+    roots (thick restart), which bounds the subspace eigensolve and the basis
+    memory at the price of more iterations (on the graphene flake 19 -> 26 at
+    480 vectors), so it is off by default.
     """
     dev = op.device
     stats = stats or DavidsonStats()
@@ -659,6 +694,11 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
             if stall >= 3 or it == maxiter - 1:
                 refine_now = True
                 break
+        lrs = int(restart or 0)
+        if lrs > 0 and it > 0 and l + 2 * k > lrs and l > 2 * k:
+            l = _collapse(sub, Rb, Lb, l, k, N, ws, st)
+            new0 = l
+            stats.restarts += 1
         sel = _precondition_select(nL, nR, k, l, kmax, conv)
         dk = len(sel)
         if dk == 0:

@@ -33,7 +33,8 @@ def test_nonhermitian_davidson_vs_casida(name, sigma):
     k = 6
     C = gs.getKSCoefficients()
     op = CasidaOperatorGPU.from_orbitals(C[:, ao], C[:, av], gs.getOverlapMatrix(), gs.orbsPerAtom, omega, g, glr, 1)
-    w, Cg, XmY, XpY, st = nonhermitian_davidson_gpu(op, k, conv=1e-10, maxiter=100)
+    # restart=0: the reference's trajectory (unbounded basis), so iteration counts compare
+    w, Cg, XmY, XpY, st = nonhermitian_davidson_gpu(op, k, conv=1e-10, maxiter=100, restart=0)
     w = w.cpu().numpy()
     assert np.abs(w - full[0][:k]).max() < 1e-10
     xpy = op.from_padded(XpY).cpu().numpy()
@@ -81,3 +82,34 @@ def test_mixed_precision_davidson(precision):
     w64 = nonhermitian_davidson_gpu(op, 8, conv=1e-6, maxiter=100)[0].cpu().numpy()
     wlo = nonhermitian_davidson_gpu(op, 8, conv=1e-5, maxiter=100, precision=precision)[0].cpu().numpy()
     assert np.abs(w64 - wlo).max() < 1e-6
+
+
+def test_thick_restart_converges_to_casida_roots():
+    """A small restart threshold collapses the basis to span(X+Y, X-Y) every few
+    iterations; the converged roots and vectors are unchanged."""
+    from dftbaby_amd.engine import CasidaOperatorGPU, nonhermitian_davidson_gpu
+    gs, ao, av, q_oo, q_ov, q_vv, omega, df, g, glr = _setup("C12H14", 1, 0.05, 11)
+    o, v = omega.shape
+    full = O.Casida(g, glr, q_oo, q_vv, q_ov, omega, 0 * omega, df, o, v)
+    k = 6
+    C = gs.getKSCoefficients()
+    op = CasidaOperatorGPU.from_orbitals(C[:, ao], C[:, av], gs.getOverlapMatrix(), gs.orbsPerAtom, omega, g, glr, 1)
+    w, Cg, XmY, XpY, st = nonhermitian_davidson_gpu(op, k, conv=1e-9, maxiter=200, restart=8 * k)
+    assert st.restarts > 0
+    assert np.abs(w.cpu().numpy() - full[0][:k]).max() < 1e-10
+    assert _sign_err(op.from_padded(XpY).cpu().numpy(), full[3][:k]) < 1e-6
+    assert _sign_err(op.from_padded(XmY).cpu().numpy(), full[2][:k]) < 1e-6
+
+
+def test_mixed_precision_refinement_reaches_fp64_tolerance():
+    """BF16x3 products cannot reach a 1e-9 residual; the solver refines with FP64
+    products from its own roots and converges to the FP64 answer."""
+    from dftbaby_amd.engine import CasidaOperatorGPU, nonhermitian_davidson_gpu
+    gs, ao, av, q_oo, q_ov, q_vv, omega, df, g, glr = _setup("C12H14", 1, 0.05, 5)
+    o, v = omega.shape
+    full = O.Casida(g, glr, q_oo, q_vv, q_ov, omega, 0 * omega, df, o, v)
+    C = gs.getKSCoefficients()
+    op = CasidaOperatorGPU.from_orbitals(C[:, ao], C[:, av], gs.getOverlapMatrix(), gs.orbsPerAtom, omega, g, glr, 1)
+    w, Cg, XmY, XpY, st = nonhermitian_davidson_gpu(op, 6, conv=1e-9, maxiter=200, precision=2)
+    assert st.refined
+    assert np.abs(w.cpu().numpy() - full[0][:6]).max() < 1e-10
