@@ -149,6 +149,37 @@ def test_products_vs_dense_buildAB(rng):
 
 
 # --------------------------------------------------------------------------- a6/a7/a8
+@pytest.mark.parametrize("name,lc", [("C8H10", 1), ("C12H14", 1)])
+def test_ao_exchange_identity(name, lc, rng):
+    """The AO-basis exchange sequence the GPU runs (four GEMMs per term over the
+    stacked [Y; C] coefficients, gamma_lr expanded to orbital pairs as a Hadamard
+    factor) equals the reference's transition-charge products exactly."""
+    gs = ground_state(name, lc, 0.05, 3)
+    f, orbe, C, S = gs.getOccupation(), gs.getKSEnergies(), gs.getKSCoefficients(), gs.getOverlapMatrix()
+    _, _, ao, av = O.active_orbitals(f, orbe)
+    Co, Cv = C[:, ao], C[:, av]
+    q_oo, q_ov, q_vv = O.trans_charges(Co, Cv, S, gs.orbsPerAtom)
+    omega, _ = O.orbital_differences(orbe, f, ao, av)
+    g, glr = gs.getGamma(), gs.getGamma_lr()
+    o, v = omega.shape
+    V = rng.standard_normal((o, v))
+    refP = O.ApBv(g, glr, q_oo, q_ov, q_vv, omega, V[:, :, None], 1)[:, :, 0]
+    refM = O.AmBv(g, glr, q_oo, q_ov, q_vv, omega, V[:, :, None], 1)[:, :, 0]
+    Yo, Yv = S @ Co, S @ Cv
+    amap = np.repeat(np.arange(len(gs.orbsPerAtom)), gs.orbsPerAtom)
+    G = glr[np.ix_(amap, amap)]
+    G2 = np.block([[G, G], [G, G]])
+    Av, Ao = np.vstack([Yv, Cv]), np.vstack([Yo, Co])
+    CYvT, W = np.vstack([Cv, Yv]).T, np.vstack([Co, Yo]).T
+    X1 = 0.25 * W @ (CYvT @ (G2 * (Ao @ (Av @ V.T).T)).T).T
+    X2 = 0.25 * W @ (CYvT @ (G2 * (Av @ (Ao @ V).T)).T).T
+    coul = np.einsum("Aia,AB,Bjb,jb->ia", q_ov, g, q_ov, V)
+    P = omega * V + 4 * coul - X1 - X2
+    M = omega * V - X1 + X2
+    assert np.abs(P - refP).max() < 1e-13 * np.abs(refP).max()
+    assert np.abs(M - refM).max() < 1e-13 * np.abs(refM).max()
+
+
 def test_casida_self_consistency():
     """Casida's own asserts (Solver.py:145-173) on an lc geometry."""
     gs = ground_state("C8H10", 1, 0.05, 2)
