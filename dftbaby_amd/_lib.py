@@ -31,7 +31,7 @@ class GemmArgs(ctypes.Structure):
                 ("sA", _l), ("sB", _l), ("sC", _l),
                 ("cdiv", _l), ("cstr", _l), ("cs2", _l),
                 ("rdiv", _l), ("rstr", _l), ("rs2", _l),
-                ("had", _vp), ("hmod", _l), ("ldh", _l),
+                ("had", _vp), ("hmod", _l), ("ldh", _l), ("hmap", _vp),
                 ("M", _i), ("N", _i), ("K", _i),
                 ("batch", _i),
                 ("ksplit", _i), ("kchunk", _i),
@@ -49,7 +49,8 @@ class CasidaOperator(ctypes.Structure):
                 ("omega", _vp), ("gamma", _vp), ("q_c", _vp),
                 ("q_oo", _vp * 2), ("q_ov", _vp * 2), ("qf_ov", _vp * 2), ("qf_vv", _vp * 2),
                 ("exchange", _i), ("npad", _i),
-                ("ao_v", _vp * 2), ("ao_o", _vp * 2), ("ao_cyvt", _vp * 2), ("ao_w", _vp * 2), ("gam_ao", _vp)]
+                ("ao_v", _vp * 2), ("ao_o", _vp * 2), ("ao_cyvt", _vp * 2), ("ao_w", _vp * 2), ("gam_ao", _vp),
+                ("amap", _vp)]
 
 
 class EnsMolecule(ctypes.Structure):

@@ -109,6 +109,10 @@ __device__ void tri_solve(const double* d, const double* e, int l, double lam, d
 // workgroup: Householder tridiagonalisation, Sturm bisection, inverse
 // iteration (clusters re-orthogonalised), back-transformation.  sv, sp: LDS
 // scratch of at least l doubles each; sh: >= blockDim/64 doubles; s_bnd: 2.
+__device__ __noinline__ void syevx_tail(int l, int nev, double* __restrict__ evals, double* __restrict__ evecs,
+                                        char* __restrict__ ws, int* __restrict__ info, double* sv, double* sp,
+                                        double* sh, double* s_bnd);
+
 __device__ __noinline__ void syevx_block(const double* __restrict__ A, int l, int ld, int nev,
                                          double* __restrict__ evals, double* __restrict__ evecs,
                                          char* __restrict__ ws, int* __restrict__ info, double* sv, double* sp,
@@ -174,6 +178,23 @@ __device__ __noinline__ void syevx_block(const double* __restrict__ A, int l, in
     for (int r = tid; r < n2; r += blockDim.x) W[(long)(j + 1 + r) * l + j] = sv[r];
     __syncthreads();
   }
+  syevx_tail(l, nev, evals, evecs, ws, info, sv, sp, sh, s_bnd);
+}
+
+// Steps after the Householder reduction of W (reflectors below the first
+// subdiagonal, d[0..l-3], e[0..l-3], tau set): last 2x2 block, bisection for
+// the nev lowest eigenvalues, inverse iteration and back-transformation.
+__device__ __noinline__ void syevx_tail(int l, int nev, double* __restrict__ evals, double* __restrict__ evecs,
+                                        char* __restrict__ ws, int* __restrict__ info, double* sv, double* sp,
+                                        double* sh, double* s_bnd) {
+  EigLayout Lo = eig_layout(l, nev);
+  double* W = (double*)(ws + Lo.W);
+  double* d = (double*)(ws + Lo.d);
+  double* e = (double*)(ws + Lo.e);
+  double* tau = (double*)(ws + Lo.tau);
+  double* Z = (double*)(ws + Lo.z);
+  double* scr = (double*)(ws + Lo.scratch);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
   if (tid == 0) {
     if (l >= 2) {
       d[l - 2] = W[(long)(l - 2) * l + (l - 2)];

@@ -201,9 +201,15 @@ struct OrthoLayout {
 
 static OrthoLayout ortho_layout(int l, int m, long n) {
   OrthoLayout o;
-  o.ks = (int)((n + 65535) / 65536);
-  if (o.ks > 64) o.ks = 64;
-  if (o.ks < 1) o.ks = 1;
+  // enough K slices to fill the GPU (>= ~2048 workgroups of 64 x 64 tiles of the l x m
+  // projection) with chunks of >= 2048 elements
+  const long tiles = (long)((l > m ? l : m) + 63) / 64 * ((m + 63) / 64);
+  long ks = (n + 2047) / 2048;
+  const long want = (2048 + tiles - 1) / tiles;
+  if (ks > want) ks = want;
+  if (ks > 512) ks = 512;
+  if (ks < 1) ks = 1;
+  o.ks = (int)ks;
   o.kchunk = ((n + o.ks - 1) / o.ks + 31) / 32 * 32;
   o.ks = (int)((n + o.kchunk - 1) / o.kchunk);
   long lm = (long)(l > m ? l : m) * m;

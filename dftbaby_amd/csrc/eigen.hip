@@ -33,13 +33,154 @@ __global__ __launch_bounds__(EIG_THREADS) void syevx_kernel(const double* __rest
   syevx_block(A, l, ld, nev, evals, evecs, ws, info, sv, sp, sh, s_bnd);
 }
 
+// ---- multi-workgroup Householder reduction (large l) -----------------------
+// Column j is reduced by two launches over all workgroups: hh_mv forms
+// p = t A22 v (one wave per row), hh_up applies A22 -= v w^T + w v^T with
+// w = p - (t/2)(p.v) v (rows spread over waves) and stores v in column j.  Every
+// workgroup rebuilds v from the still unmodified column j in LDS, and the kernel
+// boundary orders the updates, so no grid-wide synchronisation is needed.
+#define HH_THREADS 512
+
+__global__ __launch_bounds__(256) void sym_copy_kernel(const double* __restrict__ A, int l, int ld, double* __restrict__ W) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (long)l * l; i += (long)gridDim.x * blockDim.x) {
+    const long r = i / l, c = i % l;
+    W[i] = 0.5 * (A[r * ld + c] + A[c * ld + r]);
+  }
+}
+
+// Householder vector of column j in LDS: returns t (0 when the column is already
+// zero below the subdiagonal), beta via *beta_out; sv[0..n2) = v.
+__device__ double hh_vector(const double* __restrict__ W, int l, int j, double* sv, double* sh, double* beta_out) {
+  const int n2 = l - j - 1, tid = threadIdx.x;
+  double part = 0.0;
+  for (int r = tid; r < n2; r += blockDim.x) {
+    const double x = W[(long)(j + 1 + r) * l + j];
+    sv[r] = x;
+    part += x * x;
+  }
+  const double alpha2 = blk_sum(part, sh);
+  const double x0 = sv[0];
+  __syncthreads();
+  if (alpha2 <= 1e-300) {
+    *beta_out = 0.0;
+    return 0.0;
+  }
+  const double alpha = sqrt(alpha2);
+  const double beta = (x0 >= 0.0) ? -alpha : alpha;
+  const double v0 = x0 - beta;
+  const double vtv = alpha2 - x0 * x0 + v0 * v0;
+  __syncthreads();
+  if (tid == 0) sv[0] = v0;
+  __syncthreads();
+  *beta_out = beta;
+  return 2.0 / vtv;
+}
+
+__global__ __launch_bounds__(HH_THREADS) void hh_mv_kernel(const double* __restrict__ W, int l, int j,
+                                                           double* __restrict__ P) {
+  __shared__ double sv[EIG_MAXL];
+  __shared__ double sh[HH_THREADS / 64];
+  const int n2 = l - j - 1;
+  double beta;
+  const double t = hh_vector(W, l, j, sv, sh, &beta);
+  if (t == 0.0) return;
+  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int r = blockIdx.x * nw + (threadIdx.x >> 6); r < n2; r += gridDim.x * nw) {
+    const double* row = W + (long)(j + 1 + r) * l + (j + 1);
+    double s = 0.0;
+    for (int c = lane; c < n2; c += 64) s += row[c] * sv[c];
+    s = wave_sum(s);
+    if (lane == 0) P[r] = t * s;
+  }
+}
+
+// Column j keeps its original entries while the launch runs (other workgroups
+// rebuild v from it): rows r >= 1 of v equal them, and v_0 goes to hv0[j]; the
+// reflectors' first entries are written back after the last column.
+__global__ __launch_bounds__(HH_THREADS) void hh_up_kernel(double* __restrict__ W, int l, int j,
+                                                           const double* __restrict__ P, double* __restrict__ d,
+                                                           double* __restrict__ e, double* __restrict__ tau,
+                                                           double* __restrict__ hv0) {
+  __shared__ double sv[EIG_MAXL];
+  __shared__ double sw[EIG_MAXL];
+  __shared__ double sh[HH_THREADS / 64];
+  const int n2 = l - j - 1, tid = threadIdx.x;
+  double beta;
+  const double t = hh_vector(W, l, j, sv, sh, &beta);
+  if (blockIdx.x == 0 && tid == 0) {
+    tau[j] = t;
+    e[j] = beta;
+    d[j] = W[(long)j * l + j];
+    hv0[j] = (t == 0.0) ? 0.0 : sv[0];
+  }
+  const int lane = tid & 63, nw = blockDim.x >> 6;
+  if (t == 0.0) {
+    for (int r = 1 + blockIdx.x * blockDim.x + tid; r < n2; r += gridDim.x * blockDim.x)
+      W[(long)(j + 1 + r) * l + j] = 0.0;
+    return;
+  }
+  double part = 0.0;
+  for (int r = tid; r < n2; r += blockDim.x) {
+    const double pr = P[r];
+    sw[r] = pr;
+    part += pr * sv[r];
+  }
+  const double K = 0.5 * t * blk_sum(part, sh);
+  for (int r = tid; r < n2; r += blockDim.x) sw[r] -= K * sv[r];
+  __syncthreads();
+  for (int r = blockIdx.x * nw + (tid >> 6); r < n2; r += gridDim.x * nw) {
+    double* row = W + (long)(j + 1 + r) * l + (j + 1);
+    const double vr = sv[r], wr = sw[r];
+    for (int c = lane; c < n2; c += 64) row[c] -= vr * sw[c] + wr * sv[c];
+  }
+}
+
+__global__ void hh_v0_kernel(double* __restrict__ W, int l, const double* __restrict__ hv0) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j + 2 < l; j += gridDim.x * blockDim.x)
+    W[(long)(j + 1) * l + j] = hv0[j];
+}
+
+__global__ __launch_bounds__(EIG_THREADS) void syevx_tail_kernel(int l, int nev, double* __restrict__ evals,
+                                                                  double* __restrict__ evecs, char* __restrict__ ws) {
+  __shared__ double sv[EIG_MAXL];
+  __shared__ double sp[EIG_MAXL];
+  __shared__ double sh[EIG_THREADS / 64];
+  __shared__ double s_bnd[2];
+  syevx_tail(l, nev, evals, evecs, ws, (int*)nullptr, sv, sp, sh, s_bnd);
+}
+
 extern "C" int dftb_syevx(const double* A, int l, int ld, int nev, double* evals, double* evecs, void* workspace,
                           size_t ws_bytes, void* stream) {
   if (l <= 0 || nev <= 0 || nev > l || l > EIG_MAXL || ld < l || !A || !evals || !evecs || !workspace)
     return DFTB_ERR_ARG;
-  if (ws_bytes < eig_layout(l, nev).total) return DFTB_ERR_WORKSPACE;
-  hipLaunchKernelGGL(syevx_kernel, dim3(1), dim3(EIG_THREADS), 0, (hipStream_t)stream, A, l, ld, nev, evals, evecs,
-                     (char*)workspace, (int*)nullptr);
+  EigLayout Lo = eig_layout(l, nev);
+  if (ws_bytes < Lo.total) return DFTB_ERR_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  if (l < 160) {
+    hipLaunchKernelGGL(syevx_kernel, dim3(1), dim3(EIG_THREADS), 0, st, A, l, ld, nev, evals, evecs, ws,
+                       (int*)nullptr);
+    DFTB_CHECK_LAUNCH();
+    return DFTB_OK;
+  }
+  double* W = (double*)(ws + Lo.W);
+  double* d = (double*)(ws + Lo.d);
+  double* e = (double*)(ws + Lo.e);
+  double* tau = (double*)(ws + Lo.tau);
+  double* P = (double*)(ws + Lo.scratch);   // product vector (the tail's scratch is not yet in use)
+  double* hv0 = P + l;                       // first entries of the reflectors
+  hipLaunchKernelGGL(sym_copy_kernel, dim3(ceil_div((long)l * l, 256) < 4096 ? ceil_div((long)l * l, 256) : 4096),
+                     dim3(256), 0, st, A, l, ld, W);
+  for (int j = 0; j + 2 < l; ++j) {
+    const int n2 = l - j - 1;
+    const int waves = HH_THREADS / 64;
+    int g = (n2 + waves - 1) / waves;       // one wave per row
+    if (g > 256) g = 256;
+    hipLaunchKernelGGL(hh_mv_kernel, dim3(g), dim3(HH_THREADS), 0, st, (const double*)W, l, j, P);
+    hipLaunchKernelGGL(hh_up_kernel, dim3(g), dim3(HH_THREADS), 0, st, W, l, j, (const double*)P, d, e, tau, hv0);
+  }
+  hipLaunchKernelGGL(hh_v0_kernel, dim3(ceil_div(l, 256)), dim3(256), 0, st, W, l, (const double*)hv0);
+  hipLaunchKernelGGL(syevx_tail_kernel, dim3(1), dim3(EIG_THREADS), 0, st, l, nev, evals, evecs, ws);
   DFTB_CHECK_LAUNCH();
   return DFTB_OK;
 }

@@ -25,7 +25,9 @@ __device__ __forceinline__ long gemm_out_offset(const GemmArgs& p, long row, lon
 
 // Hadamard factor of element (row, col), 1 when none is set.
 __device__ __forceinline__ double gemm_had(const GemmArgs& p, long row, long col) {
-  return p.had ? p.had[(row % p.hmod) * p.ldh + (col % p.hmod)] : 1.0;
+  if (!p.had) return 1.0;
+  if (p.hmap) return p.had[(long)p.hmap[row % p.hmod] * p.ldh + p.hmap[col % p.hmod]];
+  return p.had[(row % p.hmod) * p.ldh + (col % p.hmod)];
 }
 
 static inline GemmArgs gemm_args_default() {
@@ -43,6 +45,7 @@ static inline GemmArgs gemm_args_default() {
   p.had = nullptr;
   p.hmod = 1;
   p.ldh = 0;
+  p.hmap = nullptr;
   p.M = p.N = p.K = 0;
   p.batch = 1;
   p.ksplit = 1;

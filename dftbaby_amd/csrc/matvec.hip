@@ -251,7 +251,7 @@ static int ao_gemm(int prec, const void* A, const void* A2, long lda, const void
   g.B = B; g.B2 = B2; g.ldb = ldb; g.sB = sB; g.b_trans = btrans;
   g.C = C; g.C2 = C2; g.ldc = ldc; g.sC = sC;
   g.M = M; g.N = N; g.K = K; g.batch = batch; g.alpha = alpha;
-  if (had) { g.had = op->gam_ao; g.hmod = op->npad; g.ldh = op->npad; }
+  if (had) { g.had = op->gam_ao; g.hmod = op->npad; g.ldh = op->nat + 1; g.hmap = op->amap; }
   return gemm_launch(g, st);
 }
 
@@ -323,7 +323,7 @@ static int check_op(const CasidaOperator* op) {
   if (op->precision < 0 || op->precision > 2) return DFTB_ERR_ARG;
   if (!op->omega || !op->gamma || !op->q_c) return DFTB_ERR_ARG;
   if (op->lc && op->exchange == 1) {
-    if (op->npad <= 0 || (op->npad & 7) || !op->gam_ao) return DFTB_ERR_ARG;
+    if (op->npad <= 0 || (op->npad & 7) || !op->gam_ao || !op->amap) return DFTB_ERR_ARG;
     if (!op->ao_v[0] || !op->ao_o[0] || !op->ao_cyvt[0] || !op->ao_w[0]) return DFTB_ERR_ARG;
     if (op->precision == 2 && (!op->ao_v[1] || !op->ao_o[1] || !op->ao_cyvt[1] || !op->ao_w[1])) return DFTB_ERR_ARG;
   } else if (op->lc) {

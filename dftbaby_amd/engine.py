@@ -236,11 +236,14 @@ class CasidaOperatorGPU:
         ao_cyvt[:v, :norb], ao_cyvt[:v, npad:npad + norb] = Cv.T, Yv.T
         ao_w[:o, :norb], ao_w[:o, npad:npad + norb] = Co.T, Yo.T
         amap = torch.as_tensor(np.repeat(np.arange(nat), opa), device=dev)
-        gam = zz(npad, npad)
+        # gamma_lr with a zero row/column for the padded orbitals, indexed through the atom map
+        gam = zz(nat + 1, nat + 1)
         if self.lc:
-            gam[:norb, :norb] = self.gamma_lr[amap][:, amap]
+            gam[:nat, :nat] = self.gamma_lr
+        amap_pad = torch.full((npad,), nat, dtype=torch.int32, device=dev)
+        amap_pad[:norb] = amap.to(torch.int32)
         self.npad = npad
-        self.ao = {"v": ao_v, "o": ao_o, "cyvt": ao_cyvt, "w": ao_w, "gam": gam.contiguous()}
+        self.ao = {"v": ao_v, "o": ao_o, "cyvt": ao_cyvt, "w": ao_w, "gam": gam.contiguous(), "amap": amap_pad}
         # q^{ii}_A = sum_{mu in A} C_mu,i Y_mu,i ; q^{aa}_A likewise  ->  [o][nat], [v][nat]
         self.qii = zz(nat, o).index_add_(0, amap, Co * Yo).T.contiguous()
         self.qaa = zz(nat, v).index_add_(0, amap, Cv * Yv).T.contiguous()
@@ -357,6 +360,7 @@ class CasidaOperatorGPU:
         if self.lc and self.exchange == "ao":
             s.exchange, s.npad = 1, self.npad
             s.gam_ao = self.ao["gam"].data_ptr()
+            s.amap = self.ao["amap"].data_ptr()
             if precision == 0:
                 s.q_c = self.q["c"].data_ptr()
                 for key in ("v", "o", "cyvt", "w"):

@@ -29,7 +29,8 @@ extern "C" {
  * C[b*sC + row(m) + col(n)] with row(m) = m * ldc, or (m / rdiv) * rstr +
  * (m % rdiv) * rs2 when rdiv > 0, and col(n) = n, or (n / cdiv) * cstr +
  * (n % cdiv) * cs2 when cdiv > 0.  A non-NULL had scales each result by
- * had[(m % hmod) * ldh + n % hmod] before it is stored.  ksplit > 1 splits K
+ * had[(m % hmod) * ldh + n % hmod] (with hmap: had[hmap[m % hmod] * ldh +
+ * hmap[n % hmod]]) before it is stored.  ksplit > 1 splits K
  * into chunks of kchunk and writes slice s at C + s*sCk (beta must be 0).
  * in_type GEMM_IN_BF16X3 takes (hi, lo) bf16 operand pairs in (A, A2) and
  * (B, B2) and forms hi*hi + hi*lo + lo*hi with FP32 accumulation; it needs
@@ -48,8 +49,9 @@ typedef struct GemmArgs {
   long sA, sB, sC;
   long cdiv, cstr, cs2;
   long rdiv, rstr, rs2;
-  const double* had;   /* optional Hadamard factor: element (m, n) is scaled by had[(m % hmod) * ldh + n % hmod] */
-  long hmod, ldh;
+  const double* had;   /* optional Hadamard factor: element (m, n) is scaled by had[(m % hmod) * ldh + n % hmod], */
+  long hmod, ldh;      /* or, with hmap, by had[hmap[m % hmod] * ldh + hmap[n % hmod]] */
+  const int* hmap;
   int M, N, K;
   int batch;
   int ksplit, kchunk;
@@ -131,7 +133,7 @@ typedef struct CasidaOperator {
   const void* qf_vv[2]; /* [vpad][nat][vpad] gamma_lr folded */
   /* exchange = 1: long-range exchange in the AO basis (no q_oo/q_ov/qf_*): with
    * C_o, C_v the active MO coefficients, Y = S C, npad = Norb padded to 8 and
-   * G = gamma_lr expanded to orbital pairs ([npad][npad], zero padding):
+   * G = gamma_lr on orbital pairs:
    *   ao_v    = [Y_v; C_v]        [2 npad][vpad]
    *   ao_o    = [Y_o; C_o]        [2 npad][opad]
    *   ao_cyvt = [C_v; Y_v]^T      [vpad][2 npad]
@@ -143,7 +145,8 @@ typedef struct CasidaOperator {
   const void* ao_o[2];
   const void* ao_cyvt[2];
   const void* ao_w[2];
-  const double* gam_ao;
+  const double* gam_ao;  /* gamma_lr [nat + 1][nat + 1], last row and column zero */
+  const int* amap;       /* [npad] atom of each orbital (nat for the padding) */
 } CasidaOperator;
 
 size_t dftb_casida_matvec_workspace(const CasidaOperator* op, int nvec);
