@@ -98,7 +98,7 @@ _SIGS = {
     "dftb_syevx": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _sz, _vp]),
     "dftb_unit_vectors": (_i, [_vp, _i, _l, _vp, _vp]),
     "dftb_symmetrize": (_i, [_vp, _i, _i, _vp, _vp]),
-    "dftb_coldot": (_i, [_vp, _vp, _i, _l, _vp, _vp]),
+    "dftb_coldot": (_i, [_vp, _l, _vp, _l, _i, _l, _vp, _vp]),
     "dftb_ensemble_workspace": (_sz, [ctypes.POINTER(EnsMolecule)]),
     "dftb_ensemble_solve": (_i, [_vp, _i, _d, _i, _vp]),
     "dftb_pair_features": (_i, [_vp, _vp, _vp, _vp, _vp, _l, _i, _vp, _vp]),

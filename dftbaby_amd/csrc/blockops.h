@@ -62,9 +62,12 @@ __device__ __forceinline__ double hash_unit(unsigned int a, unsigned int b) {
   return ((double)(h & 0xFFFFFF) / (double)0xFFFFFF) * 2.0 - 1.0;
 }
 
-// Solve (T - lam I) y = x in place (x -> y) with a partial-pivot LU.
-__device__ void tri_solve(const double* d, const double* e, int l, double lam, double tiny, double* u0, double* u1,
-                          double* u2, double* mlt, int* piv, double* x) {
+// LU factorisation of T - lam I with partial pivoting between rows k and k+1
+// (u0r: reciprocals of the pivots, u1/u2: the two superdiagonals of U, mlt: the
+// multipliers, piv: 1 where rows were interchanged).
+__device__ void tri_factor(const double* __restrict__ d, const double* __restrict__ e, int l, double lam,
+                           double tiny, double* __restrict__ u0r, double* __restrict__ u1, double* __restrict__ u2,
+                           double* __restrict__ mlt, int* __restrict__ piv) {
   double r0 = d[0] - lam, r1 = (l > 1) ? e[0] : 0.0;
   for (int k = 0; k < l - 1; ++k) {
     const double sub = e[k];
@@ -72,35 +75,47 @@ __device__ void tri_solve(const double* d, const double* e, int l, double lam, d
     const double b1 = (k + 1 < l - 1) ? e[k + 1] : 0.0;
     if (fabs(r0) >= fabs(sub)) {
       if (fabs(r0) < tiny) r0 = (r0 >= 0 ? tiny : -tiny);
-      double m = sub / r0;
-      u0[k] = r0; u1[k] = r1; u2[k] = 0.0; mlt[k] = m; piv[k] = 0;
+      const double m = sub / r0;
+      u0r[k] = 1.0 / r0; u1[k] = r1; u2[k] = 0.0; mlt[k] = m; piv[k] = 0;
       r0 = a1 - m * r1;
       r1 = b1;
     } else {
-      double m = r0 / sub;
-      u0[k] = sub; u1[k] = a1; u2[k] = b1; mlt[k] = m; piv[k] = 1;
+      const double m = r0 / sub;
+      u0r[k] = 1.0 / sub; u1[k] = a1; u2[k] = b1; mlt[k] = m; piv[k] = 1;
       r0 = r1 - m * a1;
       r1 = -m * b1;
     }
   }
   if (fabs(r0) < tiny) r0 = (r0 >= 0 ? tiny : -tiny);
-  u0[l - 1] = r0;
-  // forward
+  u0r[l - 1] = 1.0 / r0;
+  u1[l - 1] = 0.0;
+  u2[l - 1] = 0.0;
+}
+
+// x <- (T - lam I)^{-1} x with the factors of tri_factor.  The entries being
+// eliminated are carried in registers, so consecutive steps depend on each other
+// only through arithmetic and the loads can run ahead.
+__device__ void tri_apply(int l, const double* __restrict__ u0r, const double* __restrict__ u1,
+                          const double* __restrict__ u2, const double* __restrict__ mlt,
+                          const int* __restrict__ piv, double* __restrict__ x) {
+  double c = x[0];
   for (int k = 0; k < l - 1; ++k) {
+    const double nx = x[k + 1], m = mlt[k];
     if (piv[k]) {
-      double t = x[k];
-      x[k] = x[k + 1];
-      x[k + 1] = t - mlt[k] * x[k + 1];
+      x[k] = nx;
+      c = c - m * nx;
     } else {
-      x[k + 1] -= mlt[k] * x[k];
+      x[k] = c;
+      c = nx - m * c;
     }
   }
-  // backward
+  x[l - 1] = c;
+  double x1 = 0.0, x2 = 0.0;
   for (int k = l - 1; k >= 0; --k) {
-    double s = x[k];
-    if (k + 1 < l) s -= u1[k] * x[k + 1];
-    if (k + 2 < l) s -= u2[k] * x[k + 2];
-    x[k] = s / u0[k];
+    const double s = (x[k] - u1[k] * x1 - u2[k] * x2) * u0r[k];
+    x[k] = s;
+    x2 = x1;
+    x1 = s;
   }
 }
 
@@ -183,18 +198,18 @@ __device__ __noinline__ void syevx_block(const double* __restrict__ A, int l, in
 
 // Steps after the Householder reduction of W (reflectors below the first
 // subdiagonal, d[0..l-3], e[0..l-3], tau set): last 2x2 block, bisection for
-// the nev lowest eigenvalues, inverse iteration and back-transformation.
-__device__ __noinline__ void syevx_tail(int l, int nev, double* __restrict__ evals, double* __restrict__ evecs,
-                                        char* __restrict__ ws, int* __restrict__ info, double* sv, double* sp,
-                                        double* sh, double* s_bnd) {
+// the nev lowest eigenvalues and inverse iteration into Z (l x nev, row-major).
+// sv, sp: LDS scratch of at least l doubles each (they hold d and e).
+__device__ __noinline__ void syevx_tridiag_vectors(int l, int nev, double* __restrict__ evals,
+                                                   char* __restrict__ ws, double* sv, double* sp, double* sh,
+                                                   double* s_bnd) {
   EigLayout Lo = eig_layout(l, nev);
   double* W = (double*)(ws + Lo.W);
   double* d = (double*)(ws + Lo.d);
   double* e = (double*)(ws + Lo.e);
-  double* tau = (double*)(ws + Lo.tau);
   double* Z = (double*)(ws + Lo.z);
   double* scr = (double*)(ws + Lo.scratch);
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  const int tid = threadIdx.x;
   if (tid == 0) {
     if (l >= 2) {
       d[l - 2] = W[(long)(l - 2) * l + (l - 2)];
@@ -213,10 +228,17 @@ __device__ __noinline__ void syevx_tail(int l, int nev, double* __restrict__ eva
     s_bnd[1] = hi;
   }
   __syncthreads();
+  for (int i = tid; i < l; i += blockDim.x) {
+    sv[i] = d[i];
+    sp[i] = e[i];
+  }
+  __syncthreads();
+  const double* dl = sv;
+  const double* el = sp;
   const double lo0 = s_bnd[0], hi0 = s_bnd[1];
   const double tnorm = fmax(fabs(lo0), fabs(hi0));
   double emax2 = 0.0;
-  for (int i = 0; i + 1 < l; ++i) emax2 = fmax(emax2, e[i] * e[i]);
+  for (int i = 0; i + 1 < l; ++i) emax2 = fmax(emax2, el[i] * el[i]);
   const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax2);
   const double eps = 1.1102230246251565e-16;
 
@@ -225,7 +247,7 @@ __device__ __noinline__ void syevx_tail(int l, int nev, double* __restrict__ eva
     double a = lo0 - 2.0 * eps * tnorm - pivmin, b = hi0 + 2.0 * eps * tnorm + pivmin;
     for (int it = 0; it < 200; ++it) {
       double mid = 0.5 * (a + b);
-      if (sturm_count(d, e, l, mid, pivmin) > m) b = mid; else a = mid;
+      if (sturm_count(dl, el, l, mid, pivmin) > m) b = mid; else a = mid;
       if (b - a <= 2.0 * eps * fmax(fabs(a), fabs(b)) + pivmin) break;
     }
     evals[m] = 0.5 * (a + b);
@@ -235,27 +257,32 @@ __device__ __noinline__ void syevx_tail(int l, int nev, double* __restrict__ eva
   // ---- 3. inverse iteration.  Clusters (gap < 1e-3 ||T||) of at most EIG_SMALL_CLUSTER
   // members are handled by the thread owning their first member; larger clusters
   // are processed by the whole workgroup with classical Gram-Schmidt (twice)
-  // against the earlier members in every iteration.
+  // against the earlier members in every iteration.  Member m of a cluster
+  // starting at m0 uses the shift evals[m] + 10 (m - m0) eps ||T||; its LU
+  // factors (6 l doubles at scr + 6 l m: 1/u0, u1, u2, multipliers, pivots, x)
+  // are formed once, for all members in parallel.
   const double ortol = 1.0e-3 * tnorm;
   const double tiny = eps * fmax(tnorm, 1e-300);
   double* coef = (double*)(ws + Lo.coef);
+  for (int m = tid; m < nev; m += blockDim.x) {
+    int m0 = m;
+    while (m0 > 0 && evals[m0] - evals[m0 - 1] < ortol) --m0;
+    const double lam = evals[m] + (double)(m - m0) * 10.0 * tiny;
+    double* base = scr + (size_t)6 * l * m;
+    tri_factor(dl, el, l, lam, tiny, base, base + l, base + 2 * l, base + 3 * l, (int*)(base + 4 * l));
+  }
+  __syncthreads();
   for (int m0 = tid; m0 < nev; m0 += blockDim.x) {
     if (m0 > 0 && evals[m0] - evals[m0 - 1] < ortol) continue;  // not a cluster head
     int m1 = m0 + 1;
     while (m1 < nev && evals[m1] - evals[m1 - 1] < ortol) ++m1;
     if (m1 - m0 > EIG_SMALL_CLUSTER) continue;
-    double* base = scr + (size_t)6 * l * m0;
-    double* u0 = base;
-    double* u1 = base + l;
-    double* u2 = base + 2 * l;
-    double* mlt = base + 3 * l;
-    int* piv = (int*)(base + 4 * l);
-    double* x = base + 5 * l;
     for (int m = m0; m < m1; ++m) {
-      double lam = evals[m] + (double)(m - m0) * 10.0 * tiny;
+      double* base = scr + (size_t)6 * l * m;
+      double* x = base + 5 * l;
       for (int i = 0; i < l; ++i) x[i] = hash_unit((unsigned)m, (unsigned)i);
       for (int it = 0; it < 4; ++it) {
-        tri_solve(d, e, l, lam, tiny, u0, u1, u2, mlt, piv, x);
+        tri_apply(l, base, base + l, base + 2 * l, base + 3 * l, (const int*)(base + 4 * l), x);
         for (int mm = m0; mm < m; ++mm) {
           double s = 0.0;
           for (int i = 0; i < l; ++i) s += Z[(long)i * nev + mm] * x[i];
@@ -276,14 +303,13 @@ __device__ __noinline__ void syevx_tail(int l, int nev, double* __restrict__ eva
     int m1 = m0 + 1;
     while (m1 < nev && evals[m1] - evals[m1 - 1] < ortol) ++m1;
     if (m1 - m0 > EIG_SMALL_CLUSTER) {
-      double* base = scr + (size_t)6 * l * m0;
-      double* x = base + 5 * l;
       for (int m = m0; m < m1; ++m) {
-        const double lam = evals[m] + (double)(m - m0) * 10.0 * tiny;
+        double* base = scr + (size_t)6 * l * m;
+        double* x = base + 5 * l;
         for (int i = tid; i < l; i += blockDim.x) x[i] = hash_unit((unsigned)m, (unsigned)i);
         __syncthreads();
         for (int it = 0; it < 3; ++it) {
-          if (tid == 0) tri_solve(d, e, l, lam, tiny, base, base + l, base + 2 * l, base + 3 * l, (int*)(base + 4 * l), x);
+          if (tid == 0) tri_apply(l, base, base + l, base + 2 * l, base + 3 * l, (const int*)(base + 4 * l), x);
           __syncthreads();
           for (int pass = 0; pass < 2 && m > m0; ++pass) {
             for (int mm = m0 + tid; mm < m; mm += blockDim.x) {
@@ -313,8 +339,17 @@ __device__ __noinline__ void syevx_tail(int l, int nev, double* __restrict__ eva
     m0 = m1;
   }
   __syncthreads();
+}
 
-  // ---- 4. back-transformation z <- H_0 ... H_{l-3} z
+// ---- 4. back-transformation z <- H_0 ... H_{l-3} z in one workgroup (reflectors
+// in the columns of W below the subdiagonal), evecs = Z.
+__device__ __noinline__ void syevx_back_block(int l, int nev, double* __restrict__ evecs, char* __restrict__ ws,
+                                              double* sv) {
+  EigLayout Lo = eig_layout(l, nev);
+  const double* W = (const double*)(ws + Lo.W);
+  const double* tau = (const double*)(ws + Lo.tau);
+  double* Z = (double*)(ws + Lo.z);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
   for (int j = l - 3; j >= 0; --j) {
     const int n2 = l - j - 1;
     const double t = tau[j];
@@ -330,4 +365,11 @@ __device__ __noinline__ void syevx_tail(int l, int nev, double* __restrict__ eva
     __syncthreads();
   }
   for (long i = tid; i < (long)l * nev; i += blockDim.x) evecs[i] = Z[i];
+}
+
+__device__ __noinline__ void syevx_tail(int l, int nev, double* __restrict__ evals, double* __restrict__ evecs,
+                                        char* __restrict__ ws, int* __restrict__ info, double* sv, double* sp,
+                                        double* sh, double* s_bnd) {
+  syevx_tridiag_vectors(l, nev, evals, ws, sv, sp, sh, s_bnd);
+  syevx_back_block(l, nev, evecs, ws, sv);
 }

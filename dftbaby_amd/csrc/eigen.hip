@@ -135,18 +135,91 @@ __global__ __launch_bounds__(HH_THREADS) void hh_up_kernel(double* __restrict__ 
   }
 }
 
-__global__ void hh_v0_kernel(double* __restrict__ W, int l, const double* __restrict__ hv0) {
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j + 2 < l; j += gridDim.x * blockDim.x)
-    W[(long)(j + 1) * l + j] = hv0[j];
+// Writes the reflectors' first entries back into column j and copies every
+// reflector into row j of the upper triangle (W[j][j+1+r] = v_r), where the
+// back-transformation reads them with unit stride.  Reads touch only the strict
+// lower triangle below the subdiagonal, writes the subdiagonal and the upper
+// triangle, so the launch has no read/write overlap.
+__global__ void hh_rows_kernel(double* __restrict__ W, int l, const double* __restrict__ hv0) {
+  const long total = (long)l * l;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(idx / l), i = (int)(idx % l);
+    if (j + 2 >= l || i <= j) continue;
+    if (i == j + 1) {
+      W[(long)(j + 1) * l + j] = hv0[j];
+      W[(long)j * l + i] = hv0[j];
+    } else {
+      W[(long)j * l + i] = W[(long)i * l + j];
+    }
+  }
 }
 
-__global__ __launch_bounds__(EIG_THREADS) void syevx_tail_kernel(int l, int nev, double* __restrict__ evals,
-                                                                  double* __restrict__ evecs, char* __restrict__ ws) {
+__global__ __launch_bounds__(EIG_THREADS) void syevx_vectors_kernel(int l, int nev, double* __restrict__ evals,
+                                                                     char* __restrict__ ws) {
   __shared__ double sv[EIG_MAXL];
   __shared__ double sp[EIG_MAXL];
   __shared__ double sh[EIG_THREADS / 64];
   __shared__ double s_bnd[2];
-  syevx_tail(l, nev, evals, evecs, ws, (int*)nullptr, sv, sp, sh, s_bnd);
+  syevx_tridiag_vectors(l, nev, evals, ws, sv, sp, sh, s_bnd);
+}
+
+__global__ __launch_bounds__(EIG_THREADS) void syevx_back_kernel(int l, int nev, double* __restrict__ evecs,
+                                                                  char* __restrict__ ws) {
+  __shared__ double sv[EIG_MAXL];
+  syevx_back_block(l, nev, evecs, ws, sv);
+}
+
+// Back-transformation z <- H_0 ... H_{l-3} z with one wavefront per eigenvector:
+// entry i of the vector lives in lane i % 64, slot i / 64 (registers), reflector
+// j is read from row j of the upper triangle, and the next reflector's loads are
+// issued before the current one's reduction.  R = ceil(l / 64) slots.
+template <int R>
+__global__ __launch_bounds__(64) void hh_back_kernel(const double* __restrict__ W, int l, int nev,
+                                                     const double* __restrict__ tau, const double* __restrict__ Z,
+                                                     double* __restrict__ evecs) {
+  const int m = blockIdx.x, lane = threadIdx.x;
+  double z[R], vn[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = lane + 64 * q;
+    z[q] = (i < l) ? Z[(long)i * nev + m] : 0.0;
+  }
+  int j = l - 3;
+  double tn = 0.0;
+  if (j >= 0) {
+    tn = tau[j];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int i = lane + 64 * q;
+      vn[q] = (i > j && i < l) ? W[(long)j * l + i] : 0.0;
+    }
+  }
+  for (; j >= 0; --j) {
+    double v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = vn[q];
+    const double t = tn;
+    if (j > 0) {
+      tn = tau[j - 1];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int i = lane + 64 * q;
+        vn[q] = (i > j - 1 && i < l) ? W[(long)(j - 1) * l + i] : 0.0;
+      }
+    }
+    if (t == 0.0) continue;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) s += v[q] * z[q];
+    s = t * wave_sum(s);
+#pragma unroll
+    for (int q = 0; q < R; ++q) z[q] -= s * v[q];
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = lane + 64 * q;
+    if (i < l) evecs[(long)i * nev + m] = z[q];
+  }
 }
 
 extern "C" int dftb_syevx(const double* A, int l, int ld, int nev, double* evals, double* evecs, void* workspace,
@@ -179,8 +252,21 @@ extern "C" int dftb_syevx(const double* A, int l, int ld, int nev, double* evals
     hipLaunchKernelGGL(hh_mv_kernel, dim3(g), dim3(HH_THREADS), 0, st, (const double*)W, l, j, P);
     hipLaunchKernelGGL(hh_up_kernel, dim3(g), dim3(HH_THREADS), 0, st, W, l, j, (const double*)P, d, e, tau, hv0);
   }
-  hipLaunchKernelGGL(hh_v0_kernel, dim3(ceil_div(l, 256)), dim3(256), 0, st, W, l, (const double*)hv0);
-  hipLaunchKernelGGL(syevx_tail_kernel, dim3(1), dim3(EIG_THREADS), 0, st, l, nev, evals, evecs, ws);
+  hipLaunchKernelGGL(hh_rows_kernel, dim3(ceil_div((long)l * l, 256) < 4096 ? ceil_div((long)l * l, 256) : 4096),
+                     dim3(256), 0, st, W, l, (const double*)hv0);
+  hipLaunchKernelGGL(syevx_vectors_kernel, dim3(1), dim3(EIG_THREADS), 0, st, l, nev, evals, ws);
+  const double* Z = (const double*)(ws + Lo.z);
+  const int R = (l + 63) / 64;
+  if (R <= 4)
+    hipLaunchKernelGGL(hh_back_kernel<4>, dim3(nev), dim3(64), 0, st, (const double*)W, l, nev, (const double*)tau, Z, evecs);
+  else if (R <= 8)
+    hipLaunchKernelGGL(hh_back_kernel<8>, dim3(nev), dim3(64), 0, st, (const double*)W, l, nev, (const double*)tau, Z, evecs);
+  else if (R <= 16)
+    hipLaunchKernelGGL(hh_back_kernel<16>, dim3(nev), dim3(64), 0, st, (const double*)W, l, nev, (const double*)tau, Z, evecs);
+  else if (R <= 32)
+    hipLaunchKernelGGL(hh_back_kernel<32>, dim3(nev), dim3(64), 0, st, (const double*)W, l, nev, (const double*)tau, Z, evecs);
+  else
+    hipLaunchKernelGGL(syevx_back_kernel, dim3(1), dim3(EIG_THREADS), 0, st, l, nev, evecs, ws);
   DFTB_CHECK_LAUNCH();
   return DFTB_OK;
 }

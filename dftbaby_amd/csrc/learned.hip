@@ -17,19 +17,20 @@ static dim3 lgrid(long n) {
   return dim3((unsigned)b);
 }
 
-// cd[x] = sum_A P[A][x] * Q[A][x]   (column-wise dot over atoms; P, Q: [nat][N])
-__global__ void coldot_kernel(const double* __restrict__ P, const double* __restrict__ Q, int nat, long N,
-                              double* __restrict__ out) {
+// cd[x] = sum_A P[A][x] * Q[A][x]   (column-wise dot over atoms; P: [nat][ldp], Q: [nat][ldq])
+__global__ void coldot_kernel(const double* __restrict__ P, long ldp, const double* __restrict__ Q, long ldq,
+                              int nat, long N, double* __restrict__ out) {
   for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < N; x += (long)gridDim.x * blockDim.x) {
     double s = 0.0;
-    for (int A = 0; A < nat; ++A) s += P[(long)A * N + x] * Q[(long)A * N + x];
+    for (int A = 0; A < nat; ++A) s += P[(long)A * ldp + x] * Q[(long)A * ldq + x];
     out[x] = s;
   }
 }
 
-extern "C" int dftb_coldot(const double* P, const double* Q, int nat, long N, double* out, void* stream) {
-  if (!P || !Q || !out || nat <= 0 || N <= 0) return DFTB_ERR_ARG;
-  hipLaunchKernelGGL(coldot_kernel, lgrid(N), dim3(256), 0, (hipStream_t)stream, P, Q, nat, N, out);
+extern "C" int dftb_coldot(const double* P, long ldp, const double* Q, long ldq, int nat, long N, double* out,
+                           void* stream) {
+  if (!P || !Q || !out || nat <= 0 || N <= 0 || ldp < N || ldq < N) return DFTB_ERR_ARG;
+  hipLaunchKernelGGL(coldot_kernel, lgrid(N), dim3(256), 0, (hipStream_t)stream, P, ldp, Q, ldq, nat, N, out);
   DFTB_CHECK_LAUNCH();
   return DFTB_OK;
 }

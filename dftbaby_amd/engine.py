@@ -437,22 +437,66 @@ class CasidaOperatorGPU:
 
 
 class _Subspace:
-    """Expansion vectors and their products, preallocated on the device."""
+    """Expansion vectors and their products, preallocated on the device.  The
+    capacity grows on demand (the filled rows and the l x l projected blocks are
+    copied) up to `limit`, the memory-derived bound."""
 
-    def __init__(self, N, cap, device, nprod):
+    def __init__(self, N, cap, device, nprod, limit=None):
+        self.N, self.device = N, device
         self.cap = cap
+        self.limit = max(cap, limit or cap)
         self.bs = torch.zeros((cap, N), dtype=torch.float64, device=device)
         self.prod = [torch.zeros((cap, N), dtype=torch.float64, device=device) for _ in range(nprod)]
         self.M = [torch.zeros((cap, cap), dtype=torch.float64, device=device) for _ in range(nprod)]
         self.l = 0
 
+    def grow(self, l, need):
+        """Room for `need` vectors, keeping rows [0, l); False beyond the limit.
+        Callers drop their own references to the arrays first, so one array at a
+        time is held twice."""
+        if need <= self.cap:
+            return True
+        if need > self.limit:
+            return False
+        new = min(self.limit, max(_round_cap(need), 2 * self.cap))
+        dev, N = self.device, self.N
+
+        def rows(old):
+            t = torch.zeros((new, N), dtype=torch.float64, device=dev)
+            t[:l].copy_(old[:l])
+            return t
+
+        def square(old):
+            t = torch.zeros((new, new), dtype=torch.float64, device=dev)
+            t[:l, :l].copy_(old[:l, :l])
+            return t
+
+        self.bs = rows(self.bs)
+        for i in range(len(self.prod)):
+            self.prod[i] = rows(self.prod[i])
+        for i in range(len(self.M)):
+            self.M[i] = square(self.M[i])
+        self.cap = new
+        return True
+
+
+def _round_cap(n, q=256):
+    return ((int(n) + q - 1) // q) * q
+
 
 def _subspace_cap(N, l0, k_add, maxiter, nmat, device, frac=0.6):
+    """Memory-derived bound on the expansion space (vectors)."""
     want = l0 + k_add * maxiter
     free, total = torch.cuda.mem_get_info(device)
     free += torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)   # torch's cached blocks
     by_mem = int(frac * free / (8.0 * N * nmat))
     return max(l0 + k_add, min(want, by_mem))
+
+
+def _initial_cap(l0, k_add, limit, iters=16):
+    """First allocation: room for `iters` full expansion steps, in multiples of
+    256 vectors so that solves of one size reuse torch's cached blocks."""
+    return max(min(limit, l0 + k_add), min(limit, _round_cap(l0 + k_add * iters)))
 
 
 def _initial_indices(values, nocc, nvirt, lmax):
@@ -585,8 +629,9 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
         lmax = min(ifact * k, kmax)
     else:
         lmax = k + min(k, kmax - k)
-    cap = min(kmax, _subspace_cap(N, lmax, 2 * k, maxiter, 3, dev))
-    sub = _Subspace(N, cap, dev, 2)
+    limit = min(kmax, _subspace_cap(N, lmax, 2 * k, maxiter, 3, dev))
+    cap = _initial_cap(lmax, 2 * k, limit)
+    sub = _Subspace(N, cap, dev, 2, limit)
     bs, bp, bm = sub.bs, sub.prod[0], sub.prod[1]
     Mp, Mm = sub.M
     if initial_space is not None:
@@ -708,7 +753,16 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
         if dk == 0:
             continue
         if l + dk > cap:
-            raise ExcitedStatesNotConverged("expansion space exhausted the device memory budget (%d vectors)" % cap)
+            bs = bp = bm = Mp = Mm = None
+            if not sub.grow(l, l + dk):
+                raise ExcitedStatesNotConverged("expansion space exhausted the device memory budget (%d vectors)"
+                                                % sub.limit)
+            cap = sub.cap
+            bs, bp, bm = sub.bs, sub.prod[0], sub.prod[1]
+            Mp, Mm = sub.M
+            Tb = torch.zeros((cap, k), dtype=torch.float64, device=dev)
+            Rb = torch.zeros((cap, k), dtype=torch.float64, device=dev)
+            Lb = torch.zeros((cap, k), dtype=torch.float64, device=dev)
         wsel = torch.cat([wdev, wdev])
         seli = torch.as_tensor(sel, dtype=torch.int32, device=dev)
         Q = bs[l:l + dk]
@@ -762,8 +816,9 @@ def hermitian_davidson_gpu(op, nstates, omega_shift=None, ifact=1, conv=1.0e-14,
     sq_inv = 1.0 / sq
     om = omega * omega + 2.0 * omega * shift
     lmax = min(ifact * k, kmax)
-    cap = min(kmax, _subspace_cap(N, lmax, k, maxiter, 2, dev))
-    sub = _Subspace(N, cap, dev, 1)
+    limit = min(kmax, _subspace_cap(N, lmax, k, maxiter, 2, dev))
+    cap = _initial_cap(lmax, k, limit)
+    sub = _Subspace(N, cap, dev, 1, limit)
     bs, rbs = sub.bs, sub.prod[0]
     H = sub.M[0]
     if XpYguess is None:
@@ -841,7 +896,13 @@ def hermitian_davidson_gpu(op, nstates, omega_shift=None, ifact=1, conv=1.0e-14,
         if dk == 0:
             continue
         if l + dk > cap:
-            raise ExcitedStatesNotConverged("expansion space exhausted the device memory budget (%d vectors)" % cap)
+            bs = rbs = H = None
+            if not sub.grow(l, l + dk):
+                raise ExcitedStatesNotConverged("expansion space exhausted the device memory budget (%d vectors)"
+                                                % sub.limit)
+            cap = sub.cap
+            bs, rbs = sub.bs, sub.prod[0]
+            H = sub.M[0]
         wv = torch.sqrt(w2[:k]).contiguous()
         seli = torch.as_tensor(sel, dtype=torch.int32, device=dev)
         Q = bs[l:l + dk]

@@ -188,24 +188,28 @@ class PairMLP:
 # ---------------------------------------------------------------------------
 # features and targets
 # ---------------------------------------------------------------------------
-def _diagonal_integrals(op):
-    """(ia|ia), (ia|ia)_lr and (ii|aa)_lr on the padded pair grid (FP64), and ||q^ia||^2."""
+def _diagonal_integrals(op, chunk=1 << 17):
+    """(ia|ia), (ia|ia)_lr and (ii|aa)_lr on the padded pair grid (FP64), and ||q^ia||^2.
+    gamma q^c is formed for `chunk` pairs at a time (nat x chunk FP64 scratch)."""
     dev = op.device
     st = _stream()
     N, nat = op.N, op.nat
     qc = op.q["c"].reshape(nat, N)
-    P = torch.empty((nat, N), dtype=torch.float64, device=dev)
+    c = min(N, int(chunk))
+    P = torch.empty((nat, c), dtype=torch.float64, device=dev)
     cc = torch.empty(N, dtype=torch.float64, device=dev)
-    cl = torch.empty(N, dtype=torch.float64, device=dev)
+    cl = torch.zeros(N, dtype=torch.float64, device=dev)
     qn2 = torch.empty(N, dtype=torch.float64, device=dev)
-    gemm(op.gamma, qc, P, nat, N, nat, nat, N, N)
-    call("dftb_coldot", _p(P), _p(qc), nat, N, _p(cc), st)
-    if op.lc:
-        gemm(op.gamma_lr, qc, P, nat, N, nat, nat, N, N)
-        call("dftb_coldot", _p(P), _p(qc), nat, N, _p(cl), st)
-    else:
-        cl.zero_()
-    call("dftb_coldot", _p(qc), _p(qc), nat, N, _p(qn2), st)
+    for c0 in range(0, N, c):
+        n = min(c, N - c0)
+        q0 = qc[:, c0:]
+        gemm(op.gamma, qc, P, nat, n, nat, nat, N, c, B_off=c0)
+        call("dftb_coldot", _p(P), c, _p(q0), N, nat, n, _p(cc[c0:]), st)
+        if op.lc:
+            gemm(op.gamma_lr, qc, P, nat, n, nat, nat, N, c, B_off=c0)
+            call("dftb_coldot", _p(P), c, _p(q0), N, nat, n, _p(cl[c0:]), st)
+        call("dftb_coldot", _p(q0), N, _p(q0), N, nat, n, _p(qn2[c0:]), st)
+    del P
     xl = torch.zeros((op.opad, op.vpad), dtype=torch.float64, device=dev)
     if op.lc:
         X = torch.empty((op.nocc, nat), dtype=torch.float64, device=dev)
@@ -214,11 +218,8 @@ def _diagonal_integrals(op):
     return cc, cl, xl.reshape(-1), qn2
 
 
-def casida_diagonal_omega(op):
-    """sqrt((A+B)_ia,ia (A-B)_ia,ia) on real pairs (omega on the padding): the
-    diagonal of the Casida operator, a sharper preconditioner than omega when
-    the Coulomb and long-range exchange diagonals are large."""
-    cc, cl, xl, _ = _diagonal_integrals(op)
+def _casida_diagonal(op, ints):
+    cc, cl, xl, _ = ints
     w = op.omega
     apb = w + 4.0 * cc - xl - cl
     amb = w - xl + cl
@@ -227,13 +228,24 @@ def casida_diagonal_omega(op):
     return torch.where(mask, d, w).contiguous()
 
 
-def pair_features(op):
-    """[N][6] float32 features on the padded pair grid of a CasidaOperatorGPU:
-    omega, ||q^ia||, A_d, B_d, A_d - B_d, sqrt((A+B)_d (A-B)_d)."""
-    cc, cl, xl, qn2 = _diagonal_integrals(op)
+def _features(op, ints):
+    cc, cl, xl, qn2 = ints
     F = torch.empty((op.N, NFEAT), dtype=torch.float32, device=op.device)
     call("dftb_pair_features", _p(op.omega), _p(qn2), _p(cc), _p(cl), _p(xl), op.N, NFEAT, _p(F), _stream())
     return F
+
+
+def casida_diagonal_omega(op):
+    """sqrt((A+B)_ia,ia (A-B)_ia,ia) on real pairs (omega on the padding): the
+    diagonal of the Casida operator, a sharper preconditioner than omega when
+    the Coulomb and long-range exchange diagonals are large."""
+    return _casida_diagonal(op, _diagonal_integrals(op))
+
+
+def pair_features(op):
+    """[N][6] float32 features on the padded pair grid of a CasidaOperatorGPU:
+    omega, ||q^ia||, A_d, B_d, A_d - B_d, sqrt((A+B)_d (A-B)_d)."""
+    return _features(op, _diagonal_integrals(op))
 
 
 def real_mask(op):
@@ -282,12 +294,13 @@ def learned_start(op, model, nstates, M=None, use_shift=False, rank="model"):
     dev = op.device
     st = _stream()
     k = int(nstates)
-    F = pair_features(op)
+    ints = _diagonal_integrals(op)
+    F = _features(op, ints)
     Y = model.predict(F) if model is not None else torch.zeros((op.N, NOUT), dtype=torch.float32, device=dev)
     mask = real_mask(op)
     score = Y[:, 0].clone() if rank == "model" else -op.omega.to(torch.float32)
     score[~mask] = -1e30
-    M = int(M or max(50 * k, 256))
+    M = int(M or max(8 * k, 128))
     M = min(M, op.nocc * op.nvirt)
     sc = score.cpu().numpy()
     sel = np.argpartition(-sc, M - 1)[:M]
@@ -353,7 +366,7 @@ def learned_start(op, model, nstates, M=None, use_shift=False, rank="model"):
     cols = torch.as_tensor(sel, device=dev)
     V[:kk, cols] = XmY.T
     V[kk:, cols] = XpY.T
-    prec_omega = casida_diagonal_omega(op)
+    prec_omega = _casida_diagonal(op, ints)
     if use_shift:
         prec_omega = prec_omega + Y[:, 1].to(torch.float64) * mask.to(torch.float64)
     return V, prec_omega.contiguous(), torch.sqrt(w2.clamp_min(0))

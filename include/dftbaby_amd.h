@@ -274,7 +274,8 @@ int dftb_ensemble_solve(const EnsMolecule* mols_dev, int nmol, double conv, int 
 /* ---- learned initial space / preconditioner ---------------------------------
  * No counterpart in the reference: the engine's learned part (config of the
  * learned Davidson guess).  out[x] = sum_A P[A][x] Q[A][x] (P, Q: [nat][N]). */
-int dftb_coldot(const double* P, const double* Q, int nat, long N, double* out, void* stream);
+int dftb_coldot(const double* P, long ldp, const double* Q, long ldq, int nat, long N, double* out,
+                void* stream);
 /* Per-pair features [N][nf] (float): omega, ||q||, A_ia,ia, B_ia,ia, A-B, sqrt((A+B)(A-B)). */
 int dftb_pair_features(const double* omega, const double* qn2, const double* cc, const double* cl,
                        const double* xl, long N, int nf, float* F, void* stream);

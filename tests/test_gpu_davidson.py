@@ -113,3 +113,40 @@ def test_mixed_precision_refinement_reaches_fp64_tolerance():
     w, Cg, XmY, XpY, st = nonhermitian_davidson_gpu(op, 6, conv=1e-9, maxiter=200, precision=2)
     assert st.refined
     assert np.abs(w.cpu().numpy() - full[0][:6]).max() < 1e-10
+
+
+def test_expansion_space_grows_on_demand(monkeypatch):
+    """The expansion space starts at the smallest capacity and grows (copying the
+    filled rows and projected blocks) in both solvers; the roots are unchanged."""
+    from dftbaby_amd import engine
+    from dftbaby_amd.engine import CasidaOperatorGPU, nonhermitian_davidson_gpu, hermitian_davidson_gpu
+    gs, ao, av, q_oo, q_ov, q_vv, omega, df, g, glr = _setup("C12H14", 1, 0.05, 11)
+    o, v = omega.shape
+    full = O.Casida(g, glr, q_oo, q_vv, q_ov, omega, 0 * omega, df, o, v)
+    k = 6
+    C = gs.getKSCoefficients()
+    op = CasidaOperatorGPU.from_orbitals(C[:, ao], C[:, av], gs.getOverlapMatrix(), gs.orbsPerAtom, omega, g, glr, 1)
+    grown = []
+    orig = engine._Subspace.grow
+
+    def grow(self, l, need):
+        grown.append((self.cap, need))
+        return orig(self, l, need)
+
+    monkeypatch.setattr(engine, "_initial_cap", lambda l0, k_add, limit, iters=16: min(limit, l0 + k_add))
+    monkeypatch.setattr(engine._Subspace, "grow", grow)
+    w, Cg, XmY, XpY, st = nonhermitian_davidson_gpu(op, k, conv=1e-9, maxiter=200)
+    assert grown
+    assert np.abs(w.cpu().numpy() - full[0][:k]).max() < 1e-10
+    assert _sign_err(op.from_padded(XpY).cpu().numpy(), full[3][:k]) < 1e-6
+    # Hermitian solver (no long-range exchange)
+    gs2, ao2, av2, q_oo2, q_ov2, q_vv2, om2, df2, g2, glr2 = _setup("C20H22", 0)
+    o2, v2 = om2.shape
+    full2 = O.Casida(g2, glr2, q_oo2, q_vv2, q_ov2, om2, 0 * om2, df2, o2, v2)
+    C2 = gs2.getKSCoefficients()
+    op2 = CasidaOperatorGPU.from_orbitals(C2[:, ao2], C2[:, av2], gs2.getOverlapMatrix(), gs2.orbsPerAtom, om2, g2,
+                                          glr2, 0)
+    grown.clear()
+    w2, C2g, XmY2, XpY2, st2 = hermitian_davidson_gpu(op2, k, conv=1e-9, maxiter=200)
+    assert grown
+    assert np.abs(w2.cpu().numpy() - full2[0][:k]).max() < 1e-10

@@ -399,6 +399,32 @@ def test_syevx(l, nev, rng):
     np.testing.assert_allclose(A @ Vh, Vh * e, atol=1e-10)
 
 
+@pytest.mark.parametrize("l,nev", [(300, 20), (700, 20), (1100, 24), (2100, 8)])
+def test_syevx_clustered_wanted_roots(l, nev, rng):
+    """Subspace-like spectrum: the wanted roots lie in one cluster far below
+    ||T|| (all of them within the 1e-3 ||T|| re-orthogonalisation window), with
+    an exact degeneracy; covers the register and the workgroup back-transforms."""
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _p, _stream, Workspace
+    Q, _ = np.linalg.qr(rng.standard_normal((l, l)))
+    ev = np.concatenate([0.01 + 2e-4 * np.arange(30), np.linspace(0.5, 40.0, l - 30)])
+    ev[4] = ev[5]
+    ev = np.sort(ev)
+    A = (Q * ev) @ Q.T
+    A = 0.5 * (A + A.T)
+    Ad = T(A)
+    e_d = torch.empty(nev, dtype=torch.float64, device=_dev())
+    V = torch.empty((l, nev), dtype=torch.float64, device=_dev())
+    w, wn = Workspace(_dev()).get(_lib.load().dftb_syevx_workspace(l, nev))
+    _lib.call("dftb_syevx", _p(Ad), l, l, nev, _p(e_d), _p(V), _p(w), wn, _stream())
+    e = e_d.cpu().numpy()
+    Vh = V.cpu().numpy()
+    ref = np.linalg.eigvalsh(A)[:nev]
+    np.testing.assert_allclose(e, ref, atol=1e-12 * np.abs(ev).max())
+    np.testing.assert_allclose(Vh.T @ Vh, np.eye(nev), atol=1e-11)
+    np.testing.assert_allclose(A @ Vh, Vh * e, atol=1e-10 * np.abs(ev).max())
+
+
 def test_sqrtm(rng):
     from dftbaby_amd import _lib
     from dftbaby_amd.engine import _p, _stream, Workspace
