@@ -22,7 +22,7 @@ import torch
 
 from . import engine
 from .engine import (CasidaOperatorGPU, ExcitedStatesError, ExcitedStatesNotConverged, _p, _stream,  # noqa: F401
-                     gemm)
+                     gemm, gram)
 from .Solver import precision_code
 from . import _lib
 
@@ -32,9 +32,11 @@ EBOHR_TO_DEBYE = 2.54174623
 
 
 class LR_TDDFTB:
-    XpY = None
-    XmY = None
     Omega = None
+    # device results of the last solve; the host arrays Cij, XmY, XpY (reference
+    # layout [k][nocc][nvirt]) are copied from them on first access
+    _C_dev = _XmY_dev = _XpY_dev = None
+    _Cij_h = _XmY_h = _XpY_h = None
     # Optional learned start (a dftbaby_amd.learned.PairMLP): when set and no
     # previous solution is available as a warm start, the lc Davidson starts
     # from the model's initial space and preconditioner omega; plain Davidson
@@ -50,6 +52,23 @@ class LR_TDDFTB:
             ground_state = DFTB2(atomlist, **opts)
         self.dftb2 = ground_state
         self.timings = {}
+
+    def _host_vectors(self, name):
+        h = getattr(self, "_%s_h" % name)
+        if h is None:
+            d = getattr(self, "_%s_dev" % ("C" if name == "Cij" else name))
+            if d is not None:
+                h = self.op.from_padded(d).cpu().numpy()
+                setattr(self, "_%s_h" % name, h)
+        return h
+
+    def _set_host_vectors(self, name, value):
+        setattr(self, "_%s_h" % name, value)
+        setattr(self, "_%s_dev" % ("C" if name == "Cij" else name), None)
+
+    Cij = property(lambda self: self._host_vectors("Cij"), lambda self, v: self._set_host_vectors("Cij", v))
+    XmY = property(lambda self: self._host_vectors("XmY"), lambda self, v: self._set_host_vectors("XmY", v))
+    XpY = property(lambda self: self._host_vectors("XpY"), lambda self, v: self._set_host_vectors("XpY", v))
 
     def setGeometry(self, atomlist, **opts):
         self.dftb2.setGeometry(atomlist, **opts)
@@ -79,6 +98,7 @@ class LR_TDDFTB:
                                  diag_selected_irreps=diag_selected_irreps, ct_correction=ct_correction,
                                  stability=stability, precision=precision)
         t0 = time.time()
+        self._tdipMO_h = None
         if oszis == "Dipoles" and getattr(self.dftb2, "parameter_set", "homegrown") == "homegrown":
             self._TransitionDipoles()
         else:
@@ -249,10 +269,8 @@ class LR_TDDFTB:
         torch.cuda.synchronize()
         self._t("Davidson" if nstates is not None else "Casida", t0)
         self._C_dev, self._XmY_dev, self._XpY_dev, self._w_dev = C, XmY, XpY, w
+        self._Cij_h = self._XmY_h = self._XpY_h = None
         self.Omega = w.cpu().numpy()
-        self.Cij = op.from_padded(C).cpu().numpy()
-        self.XmY = op.from_padded(XmY).cpu().numpy()
-        self.XpY = op.from_padded(XpY).cpu().numpy()
         if diag_check == 1 and nstates is not None:
             Om_full, C_full, XmY_full, XpY_full = engine.casida_dense_gpu(op)
             Om_full = Om_full.cpu().numpy()
@@ -307,34 +325,88 @@ class LR_TDDFTB:
 
     # ----------------------------------------------------------- properties
     def _TransitionDipoles(self):
-        """d_I = sum_ia C^I_ia <i|r|a>, <p|r|q> = C^T D C (LR_TDDFTB.py:574-592)."""
+        """d_I = sum_ia C^I_ia <i|r|a> (LR_TDDFTB.py:574-592).  The dipole matrix of
+        the homegrown parameter set is D^c = S o (R^c_m + R^c_n) / 2 (R: centre of
+        each AO), so <p|r_c|q> = (P_c^T Q + Q^T P_c) / 2 with Q = S C and
+        P_c = diag(R^c) C.  The active occupied x virtual block takes two GEMMs per
+        component; the full MO matrices tdipMO_X/Y/Z are formed on first access."""
+        op = self.op
+        dev = op.device
+        gs = self.dftb2
+        if not hasattr(gs, "getOrbitalCenters"):
+            return self._TransitionDipolesDense()
+        S = torch.as_tensor(np.ascontiguousarray(gs.getOverlapMatrix(), dtype=np.float64), device=dev)
+        C = torch.as_tensor(np.ascontiguousarray(gs.getKSCoefficients(), dtype=np.float64), device=dev)
+        R = torch.as_tensor(np.ascontiguousarray(gs.getOrbitalCenters(), dtype=np.float64), device=dev)
+        norb = C.shape[0]
+        o, v = op.nocc, op.nvirt
+        Co = C[:, torch.as_tensor(self.active_occupied_orbs, device=dev)].contiguous()
+        Cv = C[:, torch.as_tensor(self.active_virtual_orbs, device=dev)].contiguous()
+        Qo = torch.empty_like(Co)
+        Qv = torch.empty_like(Cv)
+        gemm(S, Co, Qo, norb, o, norb, norb, o, o)
+        gemm(S, Cv, Qv, norb, v, norb, norb, v, v)
+        M = torch.zeros((3, op.opad, op.vpad), dtype=torch.float64, device=dev)
+        for c in range(3):
+            Po = (R[:, c:c + 1] * Co).contiguous()
+            Pv = (R[:, c:c + 1] * Cv).contiguous()
+            gemm(Po, Qv, M[c], o, v, norb, o, v, op.vpad, a_trans=1, alpha=0.5)
+            gemm(Qo, Pv, M[c], o, v, norb, o, v, op.vpad, a_trans=1, alpha=0.5, beta=1.0)
+        self._tdipMO_src = (S, C, R)
+        self._tdipMO_h = [None, None, None]
+        self._tdip_from(M.reshape(3, -1))
+        return self.tdipX, self.tdipY, self.tdipZ
+
+    def _TransitionDipolesDense(self):
+        """Same with the ground state's dense dipole matrix (getDipoleMatrix)."""
         op = self.op
         dev = op.device
         D = np.asarray(self.dftb2.getDipoleMatrix(), dtype=np.float64)
         orbs = np.asarray(self.dftb2.getKSCoefficients(), dtype=np.float64)
         norb = orbs.shape[0]
         Ct = torch.as_tensor(np.ascontiguousarray(orbs), device=dev)
-        tmo = []
         M = torch.zeros((3, op.opad, op.vpad), dtype=torch.float64, device=dev)
         ao = torch.as_tensor(self.active_occupied_orbs, device=dev)
         av = torch.as_tensor(self.active_virtual_orbs, device=dev)
+        self._tdipMO_src = None
+        self._tdipMO_h = [None, None, None]
         for c in range(3):
             Dc = torch.as_tensor(np.ascontiguousarray(D[:, :, c]), device=dev)
             X = torch.empty((norb, norb), dtype=torch.float64, device=dev)
             gemm(Dc, Ct, X, norb, norb, norb, norb, norb, norb)
             T = torch.empty((norb, norb), dtype=torch.float64, device=dev)
             gemm(Ct, X, T, norb, norb, norb, norb, norb, norb, a_trans=1)
-            tmo.append(T)
             M[c, :op.nocc, :op.nvirt] = T[ao][:, av]
-        self.tdipMO_X, self.tdipMO_Y, self.tdipMO_Z = (t.cpu().numpy() for t in tmo)
+            self._tdipMO_h[c] = T.cpu().numpy()
         self._tdip_from(M.reshape(3, -1))
         return self.tdipX, self.tdipY, self.tdipZ
+
+    def _tdipMO(self, c):
+        """<p|r_c|q> over all MOs (host, [norb][norb])."""
+        h = getattr(self, "_tdipMO_h", None)
+        if h is None:
+            raise AttributeError("transition dipoles between MOs are formed by getEnergies(oszis='Dipoles')")
+        if h[c] is None:
+            S, C, R = self._tdipMO_src
+            norb = C.shape[0]
+            Q = torch.empty_like(C)
+            gemm(S, C, Q, norb, norb, norb, norb, norb, norb)
+            P = (R[:, c:c + 1] * C).contiguous()
+            T = torch.empty_like(C)
+            gemm(P, Q, T, norb, norb, norb, norb, norb, norb, a_trans=1, alpha=0.5)
+            gemm(Q, P, T, norb, norb, norb, norb, norb, norb, a_trans=1, alpha=0.5, beta=1.0)
+            h[c] = T.cpu().numpy()
+        return h[c]
+
+    tdipMO_X = property(lambda self: self._tdipMO(0))
+    tdipMO_Y = property(lambda self: self._tdipMO(1))
+    tdipMO_Z = property(lambda self: self._tdipMO(2))
 
     def _tdip_from(self, Mflat):
         op = self.op
         k = self._C_dev.shape[0]
         out = torch.empty((k, 3), dtype=torch.float64, device=op.device)
-        gemm(self._C_dev, Mflat, out, k, 3, op.N, op.N, op.N, 3, b_trans=1, tile=64)
+        gram(self._C_dev, k, Mflat, 3, op.N, out, op.ws)     # split over the pair index
         t = out.cpu().numpy()
         self.tdipX, self.tdipY, self.tdipZ = t[:, 0].copy(), t[:, 1].copy(), t[:, 2].copy()
 
@@ -388,7 +460,7 @@ class LR_TDDFTB:
                 gemm(Ca, Dvv, Y[a], o, v, v, op.vpad, v, op.vpad, beta=1.0)
                 Y[a, :o, :v] += d0[c] * Ca[:o, :v]
             G = torch.empty((nex, nex), dtype=torch.float64, device=dev)
-            gemm(self._C_dev[:nex], Y.reshape(nex, -1), G, nex, nex, op.N, op.N, op.N, nex, b_trans=1, tile=64)
+            gram(self._C_dev, nex, Y.reshape(nex, -1), nex, op.N, G, op.ws)
             tdip[1:nex + 1, 1:nex + 1, c] = G.cpu().numpy().T
         self.tdip_exc = tdip
         return tdip

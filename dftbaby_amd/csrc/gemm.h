@@ -16,19 +16,26 @@ int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st);
 int gemm_reduce_launch(const double* P, long sP, int nsplit, double* C, long n, double alpha, double beta,
                        hipStream_t st);
 
-// Output offset of element (row, col) within one batch entry (column and row remaps).
-__device__ __forceinline__ long gemm_out_offset(const GemmArgs& p, long row, long col) {
-  const long r = (p.rdiv > 0) ? (row / p.rdiv) * p.rstr + (row % p.rdiv) * p.rs2 : row * p.ldc;
-  const long c = (p.cdiv > 0) ? (col / p.cdiv) * p.cstr + (col % p.cdiv) * p.cs2 : col;
-  return r + c;
+// Output offset of element (row, col) within one batch entry = row part + column
+// part (row remap: (row / rdiv) rstr + (row % rdiv) rs2, else row ldc; column
+// remap likewise with cdiv, cstr, cs2).
+__device__ __forceinline__ long gemm_row_part(const GemmArgs& p, long row) {
+  return (p.rdiv > 0) ? (row / p.rdiv) * p.rstr + (row % p.rdiv) * p.rs2 : row * p.ldc;
+}
+__device__ __forceinline__ long gemm_col_part(const GemmArgs& p, long col) {
+  return (p.cdiv > 0) ? (col / p.cdiv) * p.cstr + (col % p.cdiv) * p.cs2 : col;
+}
+// Row and column parts of the Hadamard index (had[row part + column part]); any
+// row / column gives an index inside the factor matrix.  Only for p.had != null.
+__device__ __forceinline__ long gemm_had_row(const GemmArgs& p, long row) {
+  const unsigned rm = (unsigned)row % (unsigned)p.hmod;
+  return (long)(p.hmap ? p.hmap[rm] : (int)rm) * p.ldh;
+}
+__device__ __forceinline__ long gemm_had_col(const GemmArgs& p, long col) {
+  const unsigned cm = (unsigned)col % (unsigned)p.hmod;
+  return p.hmap ? p.hmap[cm] : (long)cm;
 }
 
-// Hadamard factor of element (row, col), 1 when none is set.
-__device__ __forceinline__ double gemm_had(const GemmArgs& p, long row, long col) {
-  if (!p.had) return 1.0;
-  if (p.hmap) return p.had[(long)p.hmap[row % p.hmod] * p.ldh + p.hmap[col % p.hmod]];
-  return p.had[(row % p.hmod) * p.ldh + (col % p.hmod)];
-}
 
 static inline GemmArgs gemm_args_default() {
   GemmArgs p;

@@ -35,10 +35,6 @@ __device__ __forceinline__ f4_t mfma16(float a, float b, f4_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ long remap_addr(const GemmArgs& p, long row, long col) {
-  return gemm_out_offset(p, row, col);
-}
-
 __device__ __forceinline__ void store_out(const GemmArgs& p, long off, double val) {
   if (p.out_type == GEMM_OUT_F64) {
     double* C = (double*)p.C + off;
@@ -150,21 +146,35 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs p) {
     }
   }
 
+  // output offsets and Hadamard indices split into row and column parts, each
+  // formed once per row / column of this lane
   const long cbase = b * p.sC + (long)ks * p.sCk;
+  const bool had_on = p.had != nullptr;
+  long coff[TN], hcol[TN];
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const long col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
+    coff[j] = gemm_col_part(p, col);
+    hcol[j] = had_on ? gemm_had_col(p, col) : 0;
+  }
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j)
+    for (int r = 0; r < 4; ++r) {
+      // accumulator row: v_mfma_f64_16x16x4f64 holds rows (lane/16) + 4r,
+      // v_mfma_f32_16x16x4f32 rows 4(lane/16) + r (both: column lane%16)
+      const int rr = IsF64<T>::value ? ((lane >> 4) + 4 * r) : (4 * (lane >> 4) + r);
+      const long row = m0 + wm * (BM / 2) + i * 16 + rr;
+      if (row >= M) continue;
+      const long ro = cbase + gemm_row_part(p, row);
+      const long hr = had_on ? gemm_had_row(p, row) : 0;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        // accumulator row: v_mfma_f64_16x16x4f64 holds rows (lane/16) + 4r,
-        // v_mfma_f32_16x16x4f32 rows 4(lane/16) + r (both: column lane%16)
-        const int rr = IsF64<T>::value ? ((lane >> 4) + 4 * r) : (4 * (lane >> 4) + r);
-        long row = m0 + wm * (BM / 2) + i * 16 + rr;
-        long col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
-        if (row < M && col < N)
-          store_out(p, cbase + remap_addr(p, row, col), p.alpha * gemm_had(p, row, col) * (double)acc[i][j][r]);
+      for (int j = 0; j < TN; ++j) {
+        const long col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
+        if (col < N)
+          store_out(p, ro + coff[j], p.alpha * (had_on ? p.had[hr + hcol[j]] : 1.0) * (double)acc[i][j][r]);
       }
+    }
 }
 
 // Sum split-K partials: C[b][i] = alpha * sum_s P[s][b][i] + beta * C[b][i]  (dense, row-major M x N)

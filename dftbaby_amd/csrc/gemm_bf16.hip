@@ -147,17 +147,29 @@ __global__ __launch_bounds__(256) void gemm_bf3_kernel(GemmArgs p) {
 
   const long cbase = b * p.sC + (long)ks * p.sCk;
   const float alpha = (float)p.alpha;
+  const bool had_on = p.had != nullptr;
+  long coff[TN], hcol[TN];
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const long col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
+    coff[j] = gemm_col_part(p, col);
+    hcol[j] = had_on ? gemm_had_col(p, col) : 0;
+  }
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j)
+    for (int r = 0; r < 4; ++r) {
+      const long row = m0 + wm * (BM / 2) + i * 16 + 4 * (lane >> 4) + r;
+      if (row >= M) continue;
+      const long ro = cbase + gemm_row_part(p, row);
+      const long hr = had_on ? gemm_had_row(p, row) : 0;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        long row = m0 + wm * (BM / 2) + i * 16 + 4 * (lane >> 4) + r;
-        long col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
-        if (row < M && col < N)
-          store_out_bf(p, cbase + gemm_out_offset(p, row, col), alpha * (float)gemm_had(p, row, col) * acc[i][j][r]);
+      for (int j = 0; j < TN; ++j) {
+        const long col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
+        if (col < N)
+          store_out_bf(p, ro + coff[j], alpha * (had_on ? (float)p.had[hr + hcol[j]] : 1.0f) * acc[i][j][r]);
       }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -322,20 +334,43 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
 #undef P_ISSUE
 #undef P_COMPUTE
 
-  // epilogue: one straight-line pass per output type keeps the accumulators in registers
+  // epilogue: one straight-line pass per output type keeps the accumulators in
+  // registers.  The row / column parts of the output offset and of the Hadamard
+  // index are formed once per row (16 per lane) and column (4 per lane); the
+  // Hadamard factors of one row block are loaded before its stores.
   const long cbase = b * p.sC + (long)ks * p.sCk;
   const float alpha = (float)p.alpha;
-  auto addr = [&](long row, long col) -> long { return cbase + gemm_out_offset(p, row, col); };
+  const bool had_on = p.had != nullptr;
+  long roff[TM][4], hrow[TM][4], coff[4], hcol[4];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const long row = m0 + wm * (TM * 16) + i * 16 + 4 * (lane >> 4) + r;
+      roff[i][r] = cbase + gemm_row_part(p, row);
+      hrow[i][r] = had_on ? gemm_had_row(p, row) : 0;
+    }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const long col = n0 + wn * 64 + j * 16 + (lane & 15);
+    coff[j] = gemm_col_part(p, col);
+    hcol[j] = had_on ? gemm_had_col(p, col) : 0;
+  }
 #define P_EPI(STMT)                                                            \
-  _Pragma("unroll") for (int i = 0; i < TM; ++i)                               \
-  _Pragma("unroll") for (int j = 0; j < 4; ++j)                                \
-  _Pragma("unroll") for (int r = 0; r < 4; ++r) {                              \
-    const long row = m0 + wm * (TM * 16) + i * 16 + 4 * (lane >> 4) + r;       \
-    const long col = n0 + wn * 64 + j * 16 + (lane & 15);                      \
-    if (row < M && col < N) {                                                  \
-      const long off = addr(row, col);                                         \
-      const float val = alpha * (float)gemm_had(p, row, col) * acc[i][j][r];   \
-      STMT;                                                                    \
+  _Pragma("unroll") for (int i = 0; i < TM; ++i) {                             \
+    float hv_[4][4];                                                           \
+    _Pragma("unroll") for (int j = 0; j < 4; ++j)                              \
+    _Pragma("unroll") for (int r = 0; r < 4; ++r)                              \
+      hv_[j][r] = had_on ? (float)p.had[hrow[i][r] + hcol[j]] : 1.0f;          \
+    _Pragma("unroll") for (int j = 0; j < 4; ++j)                              \
+    _Pragma("unroll") for (int r = 0; r < 4; ++r) {                            \
+      const long row = m0 + wm * (TM * 16) + i * 16 + 4 * (lane >> 4) + r;     \
+      const long col = n0 + wn * 64 + j * 16 + (lane & 15);                    \
+      if (row < M && col < N) {                                                \
+        const long off = roff[i][r] + coff[j];                                 \
+        const float val = alpha * hv_[j][r] * acc[i][j][r];                    \
+        STMT;                                                                  \
+      }                                                                        \
     }                                                                          \
   }
   if (p.out_type == GEMM_OUT_F64) {

@@ -502,7 +502,15 @@ def _initial_cap(l0, k_add, limit, iters=16):
 def _initial_indices(values, nocc, nvirt, lmax):
     """Flat (i, a) indices of the lmax smallest values (stable), Solver.py:232-251."""
     flat = np.ravel(np.asarray(values))
-    idx = np.argsort(flat, kind="stable")[:lmax]
+    if 0 < lmax < flat.size:
+        # partial selection, then a stable sort of the candidates: the same indices
+        # and order as a full stable argsort (ties at the threshold by index)
+        thr = flat[np.argpartition(flat, lmax - 1)[:lmax]].max()
+        less = np.flatnonzero(flat < thr)
+        cand = np.concatenate([less, np.flatnonzero(flat == thr)[:lmax - len(less)]])
+        idx = cand[np.argsort(flat[cand], kind="stable")]
+    else:
+        idx = np.argsort(flat, kind="stable")[:lmax]
     return idx // nvirt, idx % nvirt
 
 
@@ -657,8 +665,8 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
     Tb = torch.zeros((cap, k), dtype=torch.float64, device=dev)
     Rb = torch.zeros((cap, k), dtype=torch.float64, device=dev)
     Lb = torch.zeros((cap, k), dtype=torch.float64, device=dev)
-    R = torch.empty((k, N), dtype=torch.float64, device=dev)
-    L = torch.empty((k, N), dtype=torch.float64, device=dev)
+    RL = torch.empty((2 * k, N), dtype=torch.float64, device=dev)
+    R, L = RL[:k], RL[k:]
     PR = torch.empty((k, N), dtype=torch.float64, device=dev)
     ML = torch.empty((k, N), dtype=torch.float64, device=dev)
     W = torch.empty((2 * k, N), dtype=torch.float64, device=dev)
@@ -700,8 +708,8 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
             if dg[1] > 1.0e-3:
                 raise ExcitedStatesError("(X+Y) and (X-Y) vectors not orthonormal! error = %s" % dg[1])
             # R = bs^T Rb, L = bs^T Lb and their products from the stored ones
-            gemm(Rb, bs, R, k, N, l, k, N, N, a_trans=1, tile=64)
-            gemm(Lb, bs, L, k, N, l, k, N, N, a_trans=1, tile=64)
+            RLb = torch.cat([Rb[:l], Lb[:l]], 1)          # one pass over the basis for R and L
+            gemm(RLb, bs, RL, 2 * k, N, l, 2 * k, N, N, a_trans=1, tile=64)
             gemm(Rb, bp, PR, k, N, l, k, N, N, a_trans=1, tile=64)
             gemm(Lb, bm, ML, k, N, l, k, N, N, a_trans=1, tile=64)
             w_cur = wdev

@@ -329,6 +329,11 @@ class DFTB2:
     def getGaussianOverlap(self):
         return self.gaussOmega
 
+    def getOrbitalCenters(self):
+        """[norb][3] position of the atom carrying each AO (the dipole matrix below
+        is S_mn (R_m + R_n) / 2 with these centres)."""
+        return self.pos[self.orb_atom]
+
     def getDipoleMatrix(self):
         S = self.getOverlapMatrix()
         R = self.pos[self.orb_atom]
