@@ -54,3 +54,26 @@ __device__ __forceinline__ void xcd_tile(unsigned& tx, unsigned& ty, unsigned& t
   ty = (t / nx) % ny;
   tz = t / (nx * ny);
 }
+
+// As xcd_tile, with the tiles of each z slice visited in groups of G tile rows
+// (column by column inside a group), so that the tiles one XCD runs at the same
+// time form a G x (CUs / G) block: its operand panels are fetched once per block
+// instead of once per tile row.  A bijection of [0, nx*ny*nz).
+template <unsigned G>
+__device__ __forceinline__ void xcd_tile_grouped(unsigned& tx, unsigned& ty, unsigned& tz) {
+  const unsigned nx = gridDim.x, ny = gridDim.y, nz = gridDim.z;
+  const unsigned total = nx * ny * nz;
+  const unsigned bid = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+  const unsigned q = total / 8u, r = total % 8u;
+  const unsigned xcd = bid % 8u, slot = bid / 8u;
+  const unsigned t = (xcd < r) ? xcd * (q + 1u) + slot : r * (q + 1u) + (xcd - r) * q + slot;
+  const unsigned plane = nx * ny;
+  tz = t / plane;
+  const unsigned u = t - tz * plane;
+  const unsigned g = u / (G * nx);
+  const unsigned y0 = g * G;
+  const unsigned gs = (ny - y0 < G) ? ny - y0 : G;
+  const unsigned w = u - g * G * nx;
+  ty = y0 + w % gs;
+  tx = w / gs;
+}

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs p) {
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   unsigned tx, ty, tz;
-  xcd_tile(tx, ty, tz);
+  xcd_tile_grouped<4>(tx, ty, tz);
   const long n0 = (long)tx * BN, m0 = (long)ty * BM;
   const int ks = tz % p.ksplit;
   const long b = tz / p.ksplit;

@@ -3,9 +3,15 @@
 // Each operand x is carried as a pair (hi, lo) of bf16 with hi = bf16(x) and
 // lo = bf16(x - hi), which keeps ~16 significant bits.  The product is
 //   A B ~= A_hi B_hi + A_hi B_lo + A_lo B_hi
-// with FP32 accumulation in v_mfma_f32_16x16x32_bf16 (the dropped lo*lo term
-// is below 2^-16 relative).  Three bf16 MFMAs cost 3/16 of one FP32 MFMA at the
-// MI355X dense rates (2.5 PFLOP/s bf16 vs 157 TFLOP/s fp32).
+// with FP32 accumulation in bf16 MFMAs (the dropped lo*lo term is below 2^-16
+// relative).  Three bf16 MFMAs cost 3/16 of one FP32 MFMA at the MI355X dense
+// rates (2.5 PFLOP/s bf16 vs 157 TFLOP/s fp32).
+//
+// Measured limits on gfx950 (register-only loops): v_mfma_f32_16x16x32_bf16
+// sustains about 1.46 PFLOP/s, v_mfma_f32_32x32x16_bf16 about 2.46 PFLOP/s.  The
+// pipelined kernel below is bound by its global->LDS feed (with the DMA removed
+// the same loop reaches about 74% of 2.5 PFLOP/s, with the MFMAs removed it takes
+// as long as the full kernel), which is why its tile order is grouped per XCD.
 //
 // Form: C = A B^T ("NT"): A[m][k] and B[n][k] both K-contiguous, so every lane
 // loads its 8 consecutive k values for one row with a single 16-byte LDS read,
@@ -13,6 +19,7 @@
 // k = 8*(l/16) .. +7).  Tiles: 256 threads, BM x BN = 128 x 128, BK = 32, each
 // wave a 64 x 64 block of 4x4 MFMA tiles; global->LDS staging with 16-byte
 // loads, double buffered.
+#include <stdlib.h>
 #include "common.h"
 #include "gemm.h"
 
@@ -215,11 +222,16 @@ __device__ __forceinline__ void dma16(i4_t rs, unsigned lds, unsigned off) {
 // reads have landed, then barrier
 #define P_WAIT_BARRIER(N) asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory")
 
-// NW wave64s per workgroup: NW / 2 along M, 2 along N; wave tile (256 / (NW / 2)) x 64.
-template <int NW>
+// NW wave64s per workgroup: NW / 2 along M, 2 along N; wave tile
+// (256 / (NW / 2)) x 64 computed with v_mfma_f32_32x32x16_bf16 (M32; that
+// instruction sustains about twice the rate of 16x16x32 on gfx950) or with
+// v_mfma_f32_16x16x32_bf16.
+template <int NW, bool M32>
 __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
   constexpr int WMW = NW / 2;                // waves along M
-  constexpr int TM = P_BM / WMW / 16;        // MFMA tiles per wave along M (8 or 4)
+  constexpr int WROWS = P_BM / WMW;          // rows per wave (64 or 128)
+  constexpr int TM = WROWS / 16;             // 16x16 tiles per wave along M
+  constexpr int TM32 = WROWS / 32;           // 32x32 tiles per wave along M (two along N)
   constexpr int CA = 16 / NW, CB = 8 / NW;   // 1 KB DMA chunks per wave per A / B array
   constexpr int LOADS = 2 * CA + 2 * CB;     // DMA instructions per wave per stage
   // ring of three stages, each [A hi | A lo | B hi | B lo] (48 KiB)
@@ -229,7 +241,7 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: the LDS destination goes to M0
   const int wm = wave >> 1, wn = wave & 1;
   unsigned tx, ty, tz;
-  xcd_tile(tx, ty, tz);
+  xcd_tile_grouped<4>(tx, ty, tz);   // 4 x 8 tiles (1024 x 1024) per XCD at a time
   const long n0 = (long)tx * P_BN, m0 = (long)ty * P_BM;
   const int ks = tz % p.ksplit;
   const long b = tz / p.ksplit;
@@ -264,7 +276,7 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
   const long kl = kbeg + 8 * cl;  // k of this lane's chunk at stage 0
 
 #define P_LD(rs, base, ch, off) dma16(rs, lds_addr((base) + (ch) * 512), off)
-#define P_ISSUE(kt, Ah, Al, Bh, Bl)                                                  \
+#define P_ISSUE_A(kt, Ah, Al)                                                        \
   do {                                                                               \
     const bool kok_ = kl + (long)(kt) * P_BK < kend;                                 \
     const unsigned kb_ = (unsigned)((kt) * P_BK * 2);                                \
@@ -273,23 +285,150 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
       P_LD(rah, Ah, wave * CA + i, o_);                                              \
       P_LD(ral, Al, wave * CA + i, o_);                                              \
     }                                                                                \
+  } while (0)
+#define P_ISSUE_B(kt, Bh, Bl)                                                        \
+  do {                                                                               \
+    const bool kok_ = kl + (long)(kt) * P_BK < kend;                                 \
+    const unsigned kb_ = (unsigned)((kt) * P_BK * 2);                                \
     _Pragma("unroll") for (int i = 0; i < CB; ++i) {                                 \
       const unsigned o_ = (kok_ && oB[i] != nbB) ? oB[i] + kb_ : nbB;                \
       P_LD(rbh, Bh, wave * CB + i, o_);                                              \
       P_LD(rbl, Bl, wave * CB + i, o_);                                              \
     }                                                                                \
   } while (0)
+#define P_ISSUE(kt, Ah, Al, Bh, Bl) \
+  do {                              \
+    P_ISSUE_A(kt, Ah, Al);          \
+    P_ISSUE_B(kt, Bh, Bl);          \
+  } while (0)
 
-  f4_t acc[TM][4];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = (f4_t)(0.0f);
 
-  // fragment reads: row r = 16 q + (lane & 15) holds k-chunk c = lane >> 4 in slot c ^ ((lane >> 2) & 3)
-  const int slot = ((lane >> 4) ^ ((lane >> 2) & 3)) * 8;
-  const int arow = (wm * (TM * 16) + (lane & 15)) * P_BK + slot;
-  const int brow = (wn * 64 + (lane & 15)) * P_BK + slot;
+  const int nk = (kend > kbeg) ? (int)((kend - kbeg + P_BK - 1) / P_BK) : 0;
+#define P_SLOT(st) (ring + (st) * STAGE)
+#define P_MAINLOOP(COMPUTE)                                                           \
+  if (nk > 0) {                                                                       \
+    { u16* S0 = P_SLOT(0); P_ISSUE(0, S0, S0 + SA, S0 + 2 * SA, S0 + 2 * SA + SB); }  \
+    if (nk > 1) { u16* S1 = P_SLOT(1); P_ISSUE(1, S1, S1 + SA, S1 + 2 * SA, S1 + 2 * SA + SB); } \
+    int cur = 0;                                                                      \
+    for (int kt = 0; kt < nk; ++kt) {                                                 \
+      /* stage kt is complete once at most the next stage's loads remain in flight */ \
+      if (kt + 1 < nk) P_WAIT_BARRIER(LOADS); else P_WAIT_BARRIER(0);                 \
+      if (kt + 2 < nk) {                                                              \
+        const int nx = cur == 0 ? 2 : cur - 1; /* slot of stage kt + 2 */             \
+        u16* S = P_SLOT(nx);                                                          \
+        P_ISSUE(kt + 2, S, S + SA, S + 2 * SA, S + 2 * SA + SB);                      \
+      }                                                                               \
+      const u16* C = P_SLOT(cur);                                                     \
+      COMPUTE(C, C + SA, C + 2 * SA, C + 2 * SA + SB);                                \
+      cur = cur == 2 ? 0 : cur + 1;                                                   \
+    }                                                                                 \
+  }
+  const long cbase = b * p.sC + (long)ks * p.sCk;
+  const float alpha = (float)p.alpha;
+  const bool had_on = p.had != nullptr;
+
+  if constexpr (M32) {
+    // fragment of a 32x32x16 MFMA: lane l reads row l % 32 at k-chunk 2 s + l / 32
+    // (k-step s of the stage), stored in slot chunk ^ ((row >> 2) & 3)
+    typedef float f16v_t __attribute__((ext_vector_type(16)));
+    f16v_t acc[TM32][2];
+#pragma unroll
+    for (int i = 0; i < TM32; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = (f16v_t)(0.0f);
+    const int sw = (lane >> 2) & 3;
+    const int sl0 = ((lane >> 5) ^ sw) * 8, sl1 = ((2 + (lane >> 5)) ^ sw) * 8;
+    const int arow = (wm * WROWS + (lane & 31)) * P_BK;
+    const int brow = (wn * 64 + (lane & 31)) * P_BK;
+#define P_COMPUTE32(Ah, Al, Bh, Bl)                                                              \
+  do {                                                                                           \
+    bf8_t ah_[TM32][2], al_[TM32][2], bh_[2][2], bl_[2][2];                                      \
+    _Pragma("unroll") for (int i = 0; i < TM32; ++i) {                                           \
+      ah_[i][0] = *(const bf8_t*)((Ah) + arow + i * 32 * P_BK + sl0);                            \
+      ah_[i][1] = *(const bf8_t*)((Ah) + arow + i * 32 * P_BK + sl1);                            \
+      al_[i][0] = *(const bf8_t*)((Al) + arow + i * 32 * P_BK + sl0);                            \
+      al_[i][1] = *(const bf8_t*)((Al) + arow + i * 32 * P_BK + sl1);                            \
+    }                                                                                            \
+    _Pragma("unroll") for (int j = 0; j < 2; ++j) {                                              \
+      bh_[j][0] = *(const bf8_t*)((Bh) + brow + j * 32 * P_BK + sl0);                            \
+      bh_[j][1] = *(const bf8_t*)((Bh) + brow + j * 32 * P_BK + sl1);                            \
+      bl_[j][0] = *(const bf8_t*)((Bl) + brow + j * 32 * P_BK + sl0);                            \
+      bl_[j][1] = *(const bf8_t*)((Bl) + brow + j * 32 * P_BK + sl1);                            \
+    }                                                                                            \
+    /* product-major order: independent MFMAs separate two updates of one accumulator */        \
+    _Pragma("unroll") for (int s_ = 0; s_ < 2; ++s_) {                                           \
+      _Pragma("unroll") for (int i = 0; i < TM32; ++i) _Pragma("unroll") for (int j = 0; j < 2; ++j) \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al_[i][s_], bh_[j][s_], acc[i][j], 0, 0, 0); \
+      _Pragma("unroll") for (int i = 0; i < TM32; ++i) _Pragma("unroll") for (int j = 0; j < 2; ++j) \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah_[i][s_], bl_[j][s_], acc[i][j], 0, 0, 0); \
+      _Pragma("unroll") for (int i = 0; i < TM32; ++i) _Pragma("unroll") for (int j = 0; j < 2; ++j) \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah_[i][s_], bh_[j][s_], acc[i][j], 0, 0, 0); \
+    }                                                                                            \
+  } while (0)
+    P_MAINLOOP(P_COMPUTE32)
+#undef P_COMPUTE32
+    // epilogue: accumulator entry q of tile (i, j) is row 8 (q / 4) + 4 (l / 32) + q % 4,
+    // column l % 32.  Column parts of the offsets and of the Hadamard index are formed
+    // once; each tile row block loads its Hadamard factors before its stores.
+    long coff[2], hcol[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const long col = n0 + wn * 64 + j * 32 + (lane & 31);
+      coff[j] = gemm_col_part(p, col);
+      hcol[j] = had_on ? gemm_had_col(p, col) : 0;
+    }
+#define P_EPI32(STMT)                                                                  \
+  _Pragma("unroll") for (int i = 0; i < TM32; ++i) {                                   \
+    float hv_[16][2];                                                                  \
+    _Pragma("unroll") for (int q = 0; q < 16; ++q) {                                   \
+      const long row = m0 + wm * WROWS + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3); \
+      const long hr = had_on ? gemm_had_row(p, row) : 0;                               \
+      _Pragma("unroll") for (int j = 0; j < 2; ++j)                                    \
+        hv_[q][j] = had_on ? (float)p.had[hr + hcol[j]] : 1.0f;                        \
+    }                                                                                  \
+    _Pragma("unroll") for (int q = 0; q < 16; ++q) {                                   \
+      const long row = m0 + wm * WROWS + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3); \
+      if (row < M) {                                                                   \
+        const long ro = cbase + gemm_row_part(p, row);                                 \
+        _Pragma("unroll") for (int j = 0; j < 2; ++j) {                                \
+          const long col = n0 + wn * 64 + j * 32 + (lane & 31);                        \
+          if (col < N) {                                                               \
+            const long off = ro + coff[j];                                             \
+            const float val = alpha * hv_[q][j] * acc[i][j][q];                        \
+            STMT;                                                                      \
+          }                                                                            \
+        }                                                                              \
+      }                                                                                \
+    }                                                                                  \
+  }
+    if (p.out_type == GEMM_OUT_F64) {
+      double* Cd = (double*)p.C;
+      if (p.beta != 0.0) {
+        const double beta = p.beta;
+        P_EPI32(Cd[off] = (double)val + beta * Cd[off]);
+      } else {
+        P_EPI32(Cd[off] = (double)val);
+      }
+    } else if (p.out_type == GEMM_OUT_F32) {
+      float* Cf = (float*)p.C;
+      const float beta = (float)p.beta;
+      P_EPI32(Cf[off] = beta != 0.0f ? val + beta * Cf[off] : val);
+    } else {
+      u16* Ch = (u16*)p.C;
+      u16* Cl = (u16*)p.C2;
+      P_EPI32(const u16 h = bf16_bits_rne(val); Ch[off] = h; Cl[off] = bf16_bits_rne(val - bf16_bits_to_float(h)));
+    }
+#undef P_EPI32
+  } else {
+    f4_t acc[TM][4];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = (f4_t)(0.0f);
+    // fragment reads: row r = 16 q + (lane & 15) holds k-chunk c = lane >> 4 in slot c ^ ((lane >> 2) & 3)
+    const int slot = ((lane >> 4) ^ ((lane >> 2) & 3)) * 8;
+    const int arow = (wm * WROWS + (lane & 15)) * P_BK + slot;
+    const int brow = (wn * 64 + (lane & 15)) * P_BK + slot;
 #define P_COMPUTE(Ah, Al, Bh, Bl)                                                                    \
   do {                                                                                               \
     bf8_t ah_[TM], al_[TM], bh_[4], bl_[4];                                                          \
@@ -301,7 +440,6 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
       bh_[j] = *(const bf8_t*)((Bh) + brow + j * 16 * P_BK);                                         \
       bl_[j] = *(const bf8_t*)((Bl) + brow + j * 16 * P_BK);                                         \
     }                                                                                                \
-    /* product-major order: independent MFMAs separate two updates of one accumulator */            \
     _Pragma("unroll") for (int i = 0; i < TM; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j)     \
       acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al_[i], bh_[j], acc[i][j], 0, 0, 0);       \
     _Pragma("unroll") for (int i = 0; i < TM; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j)     \
@@ -309,53 +447,23 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
     _Pragma("unroll") for (int i = 0; i < TM; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j)     \
       acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah_[i], bh_[j], acc[i][j], 0, 0, 0);       \
   } while (0)
-
-  const int nk = (kend > kbeg) ? (int)((kend - kbeg + P_BK - 1) / P_BK) : 0;
-#define P_SLOT(st) (ring + (st) * STAGE)
-  if (nk > 0) {
-    { u16* S0 = P_SLOT(0); P_ISSUE(0, S0, S0 + SA, S0 + 2 * SA, S0 + 2 * SA + SB); }
-    if (nk > 1) { u16* S1 = P_SLOT(1); P_ISSUE(1, S1, S1 + SA, S1 + 2 * SA, S1 + 2 * SA + SB); }
-    int cur = 0;
-    for (int kt = 0; kt < nk; ++kt) {
-      // stage kt is complete once at most the next stage's loads remain in flight
-      if (kt + 1 < nk) P_WAIT_BARRIER(LOADS); else P_WAIT_BARRIER(0);
-      if (kt + 2 < nk) {
-        const int nx = cur == 0 ? 2 : cur - 1;   // slot of stage kt + 2 = (kt + 2) % 3
-        u16* S = P_SLOT(nx);
-        P_ISSUE(kt + 2, S, S + SA, S + 2 * SA, S + 2 * SA + SB);
-      }
-      const u16* C = P_SLOT(cur);
-      P_COMPUTE(C, C + SA, C + 2 * SA, C + 2 * SA + SB);
-      cur = cur == 2 ? 0 : cur + 1;
-    }
-  }
-#undef P_SLOT
-#undef P_LD
-#undef P_ISSUE
+    P_MAINLOOP(P_COMPUTE)
 #undef P_COMPUTE
-
-  // epilogue: one straight-line pass per output type keeps the accumulators in
-  // registers.  The row / column parts of the output offset and of the Hadamard
-  // index are formed once per row (16 per lane) and column (4 per lane); the
-  // Hadamard factors of one row block are loaded before its stores.
-  const long cbase = b * p.sC + (long)ks * p.sCk;
-  const float alpha = (float)p.alpha;
-  const bool had_on = p.had != nullptr;
-  long roff[TM][4], hrow[TM][4], coff[4], hcol[4];
+    long roff[TM][4], hrow[TM][4], coff[4], hcol[4];
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const long row = m0 + wm * (TM * 16) + i * 16 + 4 * (lane >> 4) + r;
-      roff[i][r] = cbase + gemm_row_part(p, row);
-      hrow[i][r] = had_on ? gemm_had_row(p, row) : 0;
+      for (int r = 0; r < 4; ++r) {
+        const long row = m0 + wm * WROWS + i * 16 + 4 * (lane >> 4) + r;
+        roff[i][r] = cbase + gemm_row_part(p, row);
+        hrow[i][r] = had_on ? gemm_had_row(p, row) : 0;
+      }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long col = n0 + wn * 64 + j * 16 + (lane & 15);
+      coff[j] = gemm_col_part(p, col);
+      hcol[j] = had_on ? gemm_had_col(p, col) : 0;
     }
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const long col = n0 + wn * 64 + j * 16 + (lane & 15);
-    coff[j] = gemm_col_part(p, col);
-    hcol[j] = had_on ? gemm_had_col(p, col) : 0;
-  }
 #define P_EPI(STMT)                                                            \
   _Pragma("unroll") for (int i = 0; i < TM; ++i) {                             \
     float hv_[4][4];                                                           \
@@ -364,7 +472,7 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
       hv_[j][r] = had_on ? (float)p.had[hrow[i][r] + hcol[j]] : 1.0f;          \
     _Pragma("unroll") for (int j = 0; j < 4; ++j)                              \
     _Pragma("unroll") for (int r = 0; r < 4; ++r) {                            \
-      const long row = m0 + wm * (TM * 16) + i * 16 + 4 * (lane >> 4) + r;     \
+      const long row = m0 + wm * WROWS + i * 16 + 4 * (lane >> 4) + r;         \
       const long col = n0 + wn * 64 + j * 16 + (lane & 15);                    \
       if (row < M && col < N) {                                                \
         const long off = roff[i][r] + coff[j];                                 \
@@ -373,24 +481,31 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
       }                                                                        \
     }                                                                          \
   }
-  if (p.out_type == GEMM_OUT_F64) {
-    double* C = (double*)p.C;
-    if (p.beta != 0.0) {
-      const double beta = p.beta;
-      P_EPI(C[off] = (double)val + beta * C[off]);
+    if (p.out_type == GEMM_OUT_F64) {
+      double* Cd = (double*)p.C;
+      if (p.beta != 0.0) {
+        const double beta = p.beta;
+        P_EPI(Cd[off] = (double)val + beta * Cd[off]);
+      } else {
+        P_EPI(Cd[off] = (double)val);
+      }
+    } else if (p.out_type == GEMM_OUT_F32) {
+      float* Cf = (float*)p.C;
+      const float beta = (float)p.beta;
+      P_EPI(Cf[off] = beta != 0.0f ? val + beta * Cf[off] : val);
     } else {
-      P_EPI(C[off] = (double)val);
+      u16* Ch = (u16*)p.C;
+      u16* Cl = (u16*)p.C2;
+      P_EPI(const u16 h = bf16_bits_rne(val); Ch[off] = h; Cl[off] = bf16_bits_rne(val - bf16_bits_to_float(h)));
     }
-  } else if (p.out_type == GEMM_OUT_F32) {
-    float* C = (float*)p.C;
-    const float beta = (float)p.beta;
-    P_EPI(C[off] = beta != 0.0f ? val + beta * C[off] : val);
-  } else {
-    u16* Ch = (u16*)p.C;
-    u16* Cl = (u16*)p.C2;
-    P_EPI(const u16 h = bf16_bits_rne(val); Ch[off] = h; Cl[off] = bf16_bits_rne(val - bf16_bits_to_float(h)));
-  }
 #undef P_EPI
+  }
+#undef P_MAINLOOP
+#undef P_SLOT
+#undef P_LD
+#undef P_ISSUE
+#undef P_ISSUE_A
+#undef P_ISSUE_B
 }
 
 // wave count of the pipelined kernel (8: two waves per SIMD hide each other's stalls)
@@ -413,7 +528,13 @@ int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st) {
     dim3 grid(ceil_div(p.N, P_BN), ceil_div(p.M, P_BM), p.batch * p.ksplit);
     GemmTimer tm;
     tm.begin(st);
-    hipLaunchKernelGGL(gemm_bf3_pipe_kernel<DFTB_BF3_WAVES>, grid, dim3(DFTB_BF3_WAVES * 64), 0, st, p);
+    // 16x16x32 by default: the kernel is bound by its global->LDS feed, where the
+    // 16x16 form measured 2-4% faster; DFTB_BF3_M32=1 selects the 32x32x16 form
+    static const int m32 = getenv("DFTB_BF3_M32") ? atoi(getenv("DFTB_BF3_M32")) : 0;
+    if (m32)
+      hipLaunchKernelGGL((gemm_bf3_pipe_kernel<DFTB_BF3_WAVES, true>), grid, dim3(DFTB_BF3_WAVES * 64), 0, st, p);
+    else
+      hipLaunchKernelGGL((gemm_bf3_pipe_kernel<DFTB_BF3_WAVES, false>), grid, dim3(DFTB_BF3_WAVES * 64), 0, st, p);
     tm.end(st, GT_BF3_PIPE, 3.0 * 2.0 * p.M * p.N * (double)p.K * p.batch);
     DFTB_CHECK_LAUNCH();
     return DFTB_OK;
