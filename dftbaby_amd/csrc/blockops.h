@@ -93,29 +93,62 @@ __device__ void tri_factor(const double* __restrict__ d, const double* __restric
 }
 
 // x <- (T - lam I)^{-1} x with the factors of tri_factor.  The entries being
-// eliminated are carried in registers, so consecutive steps depend on each other
-// only through arithmetic and the loads can run ahead.
+// eliminated are carried in registers; the operands of TRI_B consecutive steps
+// are loaded together before their dependent chain, so one memory latency is
+// paid per block of steps instead of per step.
+#define TRI_B 8
 __device__ void tri_apply(int l, const double* __restrict__ u0r, const double* __restrict__ u1,
                           const double* __restrict__ u2, const double* __restrict__ mlt,
                           const int* __restrict__ piv, double* __restrict__ x) {
   double c = x[0];
-  for (int k = 0; k < l - 1; ++k) {
-    const double nx = x[k + 1], m = mlt[k];
-    if (piv[k]) {
-      x[k] = nx;
-      c = c - m * nx;
-    } else {
-      x[k] = c;
-      c = nx - m * c;
+  for (int kb = 0; kb < l - 1; kb += TRI_B) {
+    double nx[TRI_B], m[TRI_B];
+    int pv[TRI_B];
+#pragma unroll
+    for (int u = 0; u < TRI_B; ++u) {
+      const int k = kb + u;
+      const bool in = k < l - 1;
+      nx[u] = in ? x[k + 1] : 0.0;
+      m[u] = in ? mlt[k] : 0.0;
+      pv[u] = in ? piv[k] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < TRI_B; ++u) {
+      const int k = kb + u;
+      if (k < l - 1) {
+        if (pv[u]) {
+          x[k] = nx[u];
+          c = c - m[u] * nx[u];
+        } else {
+          x[k] = c;
+          c = nx[u] - m[u] * c;
+        }
+      }
     }
   }
   x[l - 1] = c;
   double x1 = 0.0, x2 = 0.0;
-  for (int k = l - 1; k >= 0; --k) {
-    const double s = (x[k] - u1[k] * x1 - u2[k] * x2) * u0r[k];
-    x[k] = s;
-    x2 = x1;
-    x1 = s;
+  for (int kb = l - 1; kb >= 0; kb -= TRI_B) {
+    double xv[TRI_B], a1[TRI_B], a2[TRI_B], r0[TRI_B];
+#pragma unroll
+    for (int u = 0; u < TRI_B; ++u) {
+      const int k = kb - u;
+      const bool in = k >= 0;
+      xv[u] = in ? x[k] : 0.0;
+      a1[u] = in ? u1[k] : 0.0;
+      a2[u] = in ? u2[k] : 0.0;
+      r0[u] = in ? u0r[k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < TRI_B; ++u) {
+      const int k = kb - u;
+      if (k >= 0) {
+        const double sv = (xv[u] - a1[u] * x1 - a2[u] * x2) * r0[u];
+        x[k] = sv;
+        x2 = x1;
+        x1 = sv;
+      }
+    }
   }
 }
 
@@ -242,15 +275,68 @@ __device__ __noinline__ void syevx_tridiag_vectors(int l, int nev, double* __res
   const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax2);
   const double eps = 1.1102230246251565e-16;
 
-  // ---- 2. bisection for the nev lowest eigenvalues
-  for (int m = tid; m < nev; m += blockDim.x) {
-    double a = lo0 - 2.0 * eps * tnorm - pivmin, b = hi0 + 2.0 * eps * tnorm + pivmin;
-    for (int it = 0; it < 200; ++it) {
-      double mid = 0.5 * (a + b);
-      if (sturm_count(dl, el, l, mid, pivmin) > m) b = mid; else a = mid;
-      if (b - a <= 2.0 * eps * fmax(fabs(a), fabs(b)) + pivmin) break;
+  // ---- 2. the nev lowest eigenvalues by Sturm counts.  With P = blockDim / nev >= 2
+  // points per eigenvalue and round (multisection), every round narrows each
+  // interval to one of its P + 1 parts; otherwise plain bisection per thread.
+  // Both stop at b - a <= 2 eps max(|a|, |b|) + pivmin.
+  // (the counts live in the 6 l nev doubles of scratch: P <= 12 l - 6 keeps them inside)
+  const int P = min((int)blockDim.x / nev, 12 * l - 6);
+  if (P < 2) {
+    for (int m = tid; m < nev; m += blockDim.x) {
+      double a = lo0 - 2.0 * eps * tnorm - pivmin, b = hi0 + 2.0 * eps * tnorm + pivmin;
+      for (int it = 0; it < 200; ++it) {
+        double mid = 0.5 * (a + b);
+        if (sturm_count(dl, el, l, mid, pivmin) > m) b = mid; else a = mid;
+        if (b - a <= 2.0 * eps * fmax(fabs(a), fabs(b)) + pivmin) break;
+      }
+      evals[m] = 0.5 * (a + b);
     }
-    evals[m] = 0.5 * (a + b);
+  } else {
+    double* ivl = scr;                          // [nev][2] intervals
+    int* cnt = (int*)(scr + 2 * nev);           // [nev][P] counts
+    int* flag = cnt + nev * P;                  // [nev] converged, then [1] all converged
+    for (int m = tid; m < nev; m += blockDim.x) {
+      ivl[2 * m] = lo0 - 2.0 * eps * tnorm - pivmin;
+      ivl[2 * m + 1] = hi0 + 2.0 * eps * tnorm + pivmin;
+      flag[m] = 0;
+    }
+    __syncthreads();
+    for (int round = 0; round < 200; ++round) {
+      if (tid < nev * P) {
+        const int m = tid / P, k = tid % P;
+        if (!flag[m]) {
+          const double a = ivl[2 * m], b = ivl[2 * m + 1];
+          const double x = a + (b - a) * (double)(k + 1) / (double)(P + 1);
+          cnt[m * P + k] = sturm_count(dl, el, l, x, pivmin);
+        }
+      }
+      __syncthreads();
+      for (int m = tid; m < nev; m += blockDim.x) {
+        if (flag[m]) continue;
+        const double a = ivl[2 * m], b = ivl[2 * m + 1];
+        double na = a, nb = b;
+        for (int k = 0; k < P; ++k) {
+          const double x = a + (b - a) * (double)(k + 1) / (double)(P + 1);
+          if (cnt[m * P + k] > m) {
+            nb = x;
+            break;
+          }
+          na = x;
+        }
+        ivl[2 * m] = na;
+        ivl[2 * m + 1] = nb;
+        if (nb - na <= 2.0 * eps * fmax(fabs(na), fabs(nb)) + pivmin) flag[m] = 1;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int all = 1;
+        for (int m = 0; m < nev; ++m) all &= flag[m];
+        flag[nev] = all;
+      }
+      __syncthreads();
+      if (flag[nev]) break;
+    }
+    for (int m = tid; m < nev; m += blockDim.x) evals[m] = 0.5 * (ivl[2 * m] + ivl[2 * m + 1]);
   }
   __syncthreads();
 
