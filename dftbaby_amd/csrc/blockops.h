@@ -156,15 +156,15 @@ __device__ void tri_apply(int l, const double* __restrict__ u0r, const double* _
 // Lowest nev eigenpairs of the symmetric l x l matrix A (leading dim ld), one
 // workgroup: Householder tridiagonalisation, Sturm bisection, inverse
 // iteration (clusters re-orthogonalised), back-transformation.  sv, sp: LDS
-// scratch of at least l doubles each; sh: >= blockDim/64 doubles; s_bnd: 2.
+// scratch of cap >= l doubles each; sh: >= blockDim/64 doubles; s_bnd: 2.
 __device__ __noinline__ void syevx_tail(int l, int nev, double* __restrict__ evals, double* __restrict__ evecs,
                                         char* __restrict__ ws, int* __restrict__ info, double* sv, double* sp,
-                                        double* sh, double* s_bnd);
+                                        double* sh, double* s_bnd, int cap);
 
 __device__ __noinline__ void syevx_block(const double* __restrict__ A, int l, int ld, int nev,
                                          double* __restrict__ evals, double* __restrict__ evecs,
                                          char* __restrict__ ws, int* __restrict__ info, double* sv, double* sp,
-                                         double* sh, double* s_bnd) {
+                                         double* sh, double* s_bnd, int cap) {
   EigLayout Lo = eig_layout(l, nev);
   double* W = (double*)(ws + Lo.W);
   double* d = (double*)(ws + Lo.d);
@@ -226,16 +226,16 @@ __device__ __noinline__ void syevx_block(const double* __restrict__ A, int l, in
     for (int r = tid; r < n2; r += blockDim.x) W[(long)(j + 1 + r) * l + j] = sv[r];
     __syncthreads();
   }
-  syevx_tail(l, nev, evals, evecs, ws, info, sv, sp, sh, s_bnd);
+  syevx_tail(l, nev, evals, evecs, ws, info, sv, sp, sh, s_bnd, cap);
 }
 
 // Steps after the Householder reduction of W (reflectors below the first
 // subdiagonal, d[0..l-3], e[0..l-3], tau set): last 2x2 block, bisection for
 // the nev lowest eigenvalues and inverse iteration into Z (l x nev, row-major).
-// sv, sp: LDS scratch of at least l doubles each (they hold d and e).
+// sv, sp: LDS scratch of cap >= l doubles each (they hold d and e).
 __device__ __noinline__ void syevx_tridiag_vectors(int l, int nev, double* __restrict__ evals,
                                                    char* __restrict__ ws, double* sv, double* sp, double* sh,
-                                                   double* s_bnd) {
+                                                   double* s_bnd, int cap) {
   EigLayout Lo = eig_layout(l, nev);
   double* W = (double*)(ws + Lo.W);
   double* d = (double*)(ws + Lo.d);
@@ -384,24 +384,43 @@ __device__ __noinline__ void syevx_tridiag_vectors(int l, int nev, double* __res
     }
   }
   __syncthreads();
-  // large clusters, cooperatively
+  // large clusters, cooperatively.  The member's factors and iterate are staged in
+  // the LDS buffers (free once the factors exist) when 4 l doubles fit, so the
+  // serial solve runs on LDS; the Gram-Schmidt coefficients take one wave each.
+  const int lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  const bool stage = 4 * l <= cap;
   for (int m0 = 0; m0 < nev;) {
     int m1 = m0 + 1;
     while (m1 < nev && evals[m1] - evals[m1 - 1] < ortol) ++m1;
     if (m1 - m0 > EIG_SMALL_CLUSTER) {
       for (int m = m0; m < m1; ++m) {
         double* base = scr + (size_t)6 * l * m;
+        const double *fu0 = base, *fu1 = base + l, *fu2 = base + 2 * l, *fml = base + 3 * l;
+        const int* fpv = (const int*)(base + 4 * l);
         double* x = base + 5 * l;
+        if (stage) {
+          for (int i = tid; i < l; i += blockDim.x) {
+            sv[i] = base[i];
+            sv[l + i] = base[l + i];
+            sv[2 * l + i] = base[2 * l + i];
+            sv[3 * l + i] = base[3 * l + i];
+            ((int*)(sp + l))[i] = ((const int*)(base + 4 * l))[i];
+          }
+          fu0 = sv; fu1 = sv + l; fu2 = sv + 2 * l; fml = sv + 3 * l;
+          fpv = (const int*)(sp + l);
+          x = sp;
+        }
         for (int i = tid; i < l; i += blockDim.x) x[i] = hash_unit((unsigned)m, (unsigned)i);
         __syncthreads();
         for (int it = 0; it < 3; ++it) {
-          if (tid == 0) tri_apply(l, base, base + l, base + 2 * l, base + 3 * l, (const int*)(base + 4 * l), x);
+          if (tid == 0) tri_apply(l, fu0, fu1, fu2, fml, fpv, x);
           __syncthreads();
           for (int pass = 0; pass < 2 && m > m0; ++pass) {
-            for (int mm = m0 + tid; mm < m; mm += blockDim.x) {
+            for (int mm = m0 + wv; mm < m; mm += nw) {
               double s = 0.0;
-              for (int i = 0; i < l; ++i) s += Z[(long)i * nev + mm] * x[i];
-              coef[mm] = s;
+              for (int i = lane; i < l; i += 64) s += Z[(long)i * nev + mm] * x[i];
+              s = wave_sum(s);
+              if (lane == 0) coef[mm] = s;
             }
             __syncthreads();
             for (int i = tid; i < l; i += blockDim.x) {
@@ -455,7 +474,7 @@ __device__ __noinline__ void syevx_back_block(int l, int nev, double* __restrict
 
 __device__ __noinline__ void syevx_tail(int l, int nev, double* __restrict__ evals, double* __restrict__ evecs,
                                         char* __restrict__ ws, int* __restrict__ info, double* sv, double* sp,
-                                        double* sh, double* s_bnd) {
-  syevx_tridiag_vectors(l, nev, evals, ws, sv, sp, sh, s_bnd);
+                                        double* sh, double* s_bnd, int cap) {
+  syevx_tridiag_vectors(l, nev, evals, ws, sv, sp, sh, s_bnd, cap);
   syevx_back_block(l, nev, evecs, ws, sv);
 }

@@ -30,7 +30,7 @@ __global__ __launch_bounds__(EIG_THREADS) void syevx_kernel(const double* __rest
   __shared__ double sp[EIG_MAXL];
   __shared__ double sh[EIG_THREADS / 64];
   __shared__ double s_bnd[2];
-  syevx_block(A, l, ld, nev, evals, evecs, ws, info, sv, sp, sh, s_bnd);
+  syevx_block(A, l, ld, nev, evals, evecs, ws, info, sv, sp, sh, s_bnd, EIG_MAXL);
 }
 
 // ---- multi-workgroup Householder reduction (large l) -----------------------
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(EIG_THREADS) void syevx_vectors_kernel(int l, int n
   __shared__ double sp[EIG_MAXL];
   __shared__ double sh[EIG_THREADS / 64];
   __shared__ double s_bnd[2];
-  syevx_tridiag_vectors(l, nev, evals, ws, sv, sp, sh, s_bnd);
+  syevx_tridiag_vectors(l, nev, evals, ws, sv, sp, sh, s_bnd, EIG_MAXL);
 }
 
 __global__ __launch_bounds__(EIG_THREADS) void syevx_back_kernel(int l, int nev, double* __restrict__ evecs,

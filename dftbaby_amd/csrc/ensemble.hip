@@ -430,7 +430,7 @@ __device__ void ens_davidson(const EnsMolecule& Mo, double conv, int maxiter, do
       break;
     }
     const int nev = k < l ? k : l;
-    syevx_block(H, l, l, nev, w2, Tb, ws + L.eig, nullptr, sv, sp, sh, bnd);
+    syevx_block(H, l, l, nev, w2, Tb, ws + L.eig, nullptr, sv, sp, sh, bnd, ENS_MAXL);
     __syncthreads();
     int bad = 0;
     for (int n = threadIdx.x; n < nev; n += blockDim.x)
