@@ -196,7 +196,7 @@ static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct OrthoLayout {
   int ks;
   long kchunk;
-  size_t part, C, G, L, Linv, ref, tmp, total;
+  size_t part, C, G, L, Linv, ref, flag, tmp, total;
 };
 
 static OrthoLayout ortho_layout(int l, int m, long n) {
@@ -220,6 +220,7 @@ static OrthoLayout ortho_layout(int l, int m, long n) {
   o.L = off; off += al256((size_t)m * m * 8);
   o.Linv = off; off += al256((size_t)m * m * 8);
   o.ref = off; off += al256((size_t)m * 8);
+  o.flag = off; off += al256(sizeof(int));
   o.tmp = off; off += al256((size_t)m * n * 8);
   o.total = off;
   return o;
@@ -241,6 +242,24 @@ static int gram(const double* X, int P, const double* Y, int Q, long n, const Or
   return gemm_reduce_launch((double*)(ws + o.part), (long)P * Q, o.ks, out, (long)P * Q, 1.0, 0.0, st);
 }
 
+// ref[j] = sqrt(G_jj)
+__global__ void diag_sqrt_kernel(const double* __restrict__ G, int m, double* __restrict__ ref) {
+  for (int j = threadIdx.x; j < m; j += blockDim.x) ref[j] = sqrt(fmax(G[(long)j * m + j], 0.0));
+}
+
+// flag = 1 when some vector kept less than half of its squared norm through the
+// projection (G_jj < ref_j^2 / 2): the Daniel-Gragg-Kaufman-Stewart test for a
+// second classical Gram-Schmidt pass
+__global__ void dgks_kernel(const double* __restrict__ G, int m, const double* __restrict__ ref, int* __restrict__ flag) {
+  __shared__ int any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < m; j += blockDim.x)
+    if (G[(long)j * m + j] < 0.5 * ref[j] * ref[j]) any = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) flag[0] = any;
+}
+
 extern "C" int dftb_orthonormalize(const double* basis, int l, double* Q, int m, long n, double tol, int* keep,
                                    void* workspace, size_t ws_bytes, void* stream) {
   if (m <= 0) return DFTB_OK;
@@ -254,12 +273,18 @@ extern "C" int dftb_orthonormalize(const double* basis, int l, double* Q, int m,
   double* L = (double*)(ws + o.L);
   double* Linv = (double*)(ws + o.Linv);
   double* ref = (double*)(ws + o.ref);
+  int* flag = (int*)(ws + o.flag);
   double* tmp = (double*)(ws + o.tmp);
   int rc;
   hipLaunchKernelGGL(init_keep_kernel, dim3((m + 255) / 256), dim3(256), 0, st, keep, m);
-  hipLaunchKernelGGL(row_norms_kernel, dim3(m), dim3(1024), 0, st, Q, n, ref);
   DFTB_CHECK_LAUNCH();
-  // two passes of classical Gram-Schmidt against the basis
+  // reference norms before the projections: the diagonal of Q Q^T
+  if ((rc = gram(Q, m, Q, m, n, o, ws, G, st))) return rc;
+  hipLaunchKernelGGL(diag_sqrt_kernel, dim3(1), dim3(256), 0, st, (const double*)G, m, ref);
+  DFTB_CHECK_LAUNCH();
+  // classical Gram-Schmidt against the basis; a second pass when the DGKS test
+  // asks for it (G = Q Q^T of the projected block serves the test and the first
+  // Cholesky-QR)
   for (int pass = 0; pass < 2 && l > 0; ++pass) {
     if ((rc = gram(basis, l, Q, m, n, o, ws, C, st))) return rc;  // C[p][j] = <basis_p, Q_j>
     GemmArgs u = gemm_args_default();
@@ -268,21 +293,30 @@ extern "C" int dftb_orthonormalize(const double* basis, int l, double* Q, int m,
     u.C = Q; u.ldc = n;
     u.M = m; u.N = (int)n; u.K = l; u.alpha = -1.0; u.beta = 1.0; u.tile = 64;
     if ((rc = gemm_launch(u, st))) return rc;
-  }
-  // two passes of Cholesky-QR inside the block
-  for (int pass = 0; pass < 2; ++pass) {
     if ((rc = gram(Q, m, Q, m, n, o, ws, G, st))) return rc;
+    if (pass == 0) {
+      hipLaunchKernelGGL(dgks_kernel, dim3(1), dim3(64), 0, st, (const double*)G, m, (const double*)ref, flag);
+      DFTB_CHECK_LAUNCH();
+      int hflag = 1;
+      if (hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return DFTB_ERR_LAUNCH;
+      if (hipStreamSynchronize(st) != hipSuccess) return DFTB_ERR_LAUNCH;
+      if (!hflag) break;
+    }
+  }
+  // two passes of Cholesky-QR inside the block: Q -> tmp -> Q
+  for (int pass = 0; pass < 2; ++pass) {
+    const double* src = pass == 0 ? Q : tmp;
+    double* dst = pass == 0 ? tmp : Q;
+    if (pass == 1 && (rc = gram(src, m, src, m, n, o, ws, G, st))) return rc;
     hipLaunchKernelGGL(cholqr_kernel, dim3(1), dim3(256), 0, st, G, m, pass == 0 ? ref : (const double*)nullptr,
                        pass == 0 ? tol * tol : 1.0e-24, L, Linv, keep);
     DFTB_CHECK_LAUNCH();
     GemmArgs u = gemm_args_default();
     u.A = Linv; u.lda = m;
-    u.B = Q; u.ldb = n;
-    u.C = tmp; u.ldc = n;
+    u.B = src; u.ldb = n;
+    u.C = dst; u.ldc = n;
     u.M = m; u.N = (int)n; u.K = m; u.tile = 64;
     if ((rc = gemm_launch(u, st))) return rc;
-    if (hipMemcpyAsync(Q, tmp, (size_t)m * n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
-      return DFTB_ERR_LAUNCH;
   }
   return DFTB_OK;
 }
