@@ -53,16 +53,22 @@ __device__ __forceinline__ void store_out(const GemmArgs& p, long off, double va
   }
 }
 
-template <typename T, int BM, int BN, int BK, bool AT, bool BT>
+// WGM x (4 / WGM) waves over the BM x BN tile (2 x 2 for the square tiles; 1 x 4
+// and 4 x 1 for the 32- and 48-wide tiles of the tall-skinny basis products).
+template <typename T, int BM, int BN, int BK, bool AT, bool BT, int WGM = 2>
 __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs p) {
   typedef typename AccT<T>::type V4;
   __shared__ T As[2][BK][BM + 1];
   __shared__ T Bs[2][BK][BN + 1];
-  constexpr int TM = BM / 32, TN = BN / 32;
+  constexpr int WGN = 4 / WGM;
+  constexpr int WBM = BM / WGM, WBN = BN / WGN;  // wave tile
+  constexpr int TM = WBM / 16, TN = WBN / 16;
+  static_assert(TM * 16 * WGM == BM && TN * 16 * WGN == BN, "tile / wave layout");
   constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+  static_assert(NA * 256 == BM * BK && NB * 256 == BN * BK, "tile / thread layout");
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WGN, wn = wave % WGN;
   unsigned tx, ty, tz;
   xcd_tile_grouped<4>(tx, ty, tz);
   const long n0 = (long)tx * BN, m0 = (long)ty * BM;
@@ -133,9 +139,9 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs p) {
         T af[TM], bf[TN];
         const int kr = kk + (lane >> 4);
 #pragma unroll
-        for (int i = 0; i < TM; ++i) af[i] = As[cur][kr][wm * (BM / 2) + i * 16 + (lane & 15)];
+        for (int i = 0; i < TM; ++i) af[i] = As[cur][kr][wm * WBM + i * 16 + (lane & 15)];
 #pragma unroll
-        for (int j = 0; j < TN; ++j) bf[j] = Bs[cur][kr][wn * (BN / 2) + j * 16 + (lane & 15)];
+        for (int j = 0; j < TN; ++j) bf[j] = Bs[cur][kr][wn * WBN + j * 16 + (lane & 15)];
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -153,7 +159,7 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs p) {
   long coff[TN], hcol[TN];
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
-    const long col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
+    const long col = n0 + wn * WBN + j * 16 + (lane & 15);
     coff[j] = gemm_col_part(p, col);
     hcol[j] = had_on ? gemm_had_col(p, col) : 0;
   }
@@ -164,13 +170,13 @@ __global__ __launch_bounds__(256) void gemm_fp_kernel(GemmArgs p) {
       // accumulator row: v_mfma_f64_16x16x4f64 holds rows (lane/16) + 4r,
       // v_mfma_f32_16x16x4f32 rows 4(lane/16) + r (both: column lane%16)
       const int rr = IsF64<T>::value ? ((lane >> 4) + 4 * r) : (4 * (lane >> 4) + r);
-      const long row = m0 + wm * (BM / 2) + i * 16 + rr;
+      const long row = m0 + wm * WBM + i * 16 + rr;
       if (row >= M) continue;
       const long ro = cbase + gemm_row_part(p, row);
       const long hr = had_on ? gemm_had_row(p, row) : 0;
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        const long col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
+        const long col = n0 + wn * WBN + j * 16 + (lane & 15);
         if (col < N)
           store_out(p, ro + coff[j], p.alpha * (had_on ? p.had[hr + hcol[j]] : 1.0) * (double)acc[i][j][r]);
       }
@@ -188,13 +194,13 @@ __global__ void gemm_reduce_kernel(const double* P, long sP, int nsplit, double*
   C[i] = s;
 }
 
-template <typename T, int BM, int BN, int BK>
+template <typename T, int BM, int BN, int BK, int WGM = 2>
 static void launch_fp(const GemmArgs& p, hipStream_t st) {
   dim3 grid(ceil_div(p.N, BN), ceil_div(p.M, BM), p.batch * p.ksplit);
-  if (!p.a_trans && !p.b_trans) hipLaunchKernelGGL((gemm_fp_kernel<T, BM, BN, BK, false, false>), grid, dim3(256), 0, st, p);
-  else if (p.a_trans && !p.b_trans) hipLaunchKernelGGL((gemm_fp_kernel<T, BM, BN, BK, true, false>), grid, dim3(256), 0, st, p);
-  else if (!p.a_trans && p.b_trans) hipLaunchKernelGGL((gemm_fp_kernel<T, BM, BN, BK, false, true>), grid, dim3(256), 0, st, p);
-  else hipLaunchKernelGGL((gemm_fp_kernel<T, BM, BN, BK, true, true>), grid, dim3(256), 0, st, p);
+  if (!p.a_trans && !p.b_trans) hipLaunchKernelGGL((gemm_fp_kernel<T, BM, BN, BK, false, false, WGM>), grid, dim3(256), 0, st, p);
+  else if (p.a_trans && !p.b_trans) hipLaunchKernelGGL((gemm_fp_kernel<T, BM, BN, BK, true, false, WGM>), grid, dim3(256), 0, st, p);
+  else if (!p.a_trans && p.b_trans) hipLaunchKernelGGL((gemm_fp_kernel<T, BM, BN, BK, false, true, WGM>), grid, dim3(256), 0, st, p);
+  else hipLaunchKernelGGL((gemm_fp_kernel<T, BM, BN, BK, true, true, WGM>), grid, dim3(256), 0, st, p);
 }
 
 // ---------------------------------------------------------------------------
@@ -277,7 +283,14 @@ int gemm_launch(const GemmArgs& p0, hipStream_t st) {
   GemmTimer tm;
   tm.begin(st);
   if (p.in_type == GEMM_IN_F64) {
-    if (small) launch_fp<double, 64, 64, 16>(p, st);
+    // tall-skinny products of the Davidson basis (one side 20-40 vectors, the other
+    // hundreds of basis vectors or the pair index): narrow tiles cut the padded
+    // MFMA work, which bounds these calls at the FP64 MFMA rate
+    if (small && p.M <= 32 && p.N > 48) launch_fp<double, 32, 64, 16, 1>(p, st);
+    else if (small && p.M <= 48 && p.N > 48) launch_fp<double, 48, 64, 16, 1>(p, st);
+    else if (small && p.N <= 32 && p.M > 48) launch_fp<double, 64, 32, 16, 4>(p, st);
+    else if (small && p.N <= 48 && p.M > 48) launch_fp<double, 64, 48, 16, 4>(p, st);
+    else if (small) launch_fp<double, 64, 64, 16>(p, st);
     else launch_fp<double, 128, 128, 16>(p, st);
   } else {
     if (small) launch_fp<float, 64, 64, 32>(p, st);

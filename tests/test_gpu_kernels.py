@@ -38,6 +38,32 @@ def test_gemm_f64(at, bt, M, N, K, rng):
     np.testing.assert_allclose(C.cpu().numpy(), ref, rtol=1e-12, atol=1e-11)
 
 
+@pytest.mark.parametrize("at,bt", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(20, 300, 70), (33, 130, 17), (40, 1000, 300), (300, 20, 250), (200, 45, 33),
+                                   (700, 40, 3000)])
+def test_gemm_f64_narrow_tiles(at, bt, M, N, K, rng):
+    """The 32/48-wide tiles used when one side is 20-48 (Davidson basis products),
+    including ragged edges and the edge rows/columns of partially filled tiles."""
+    from dftbaby_amd.engine import gemm
+    A = rng.standard_normal((K, M) if at else (M, K))
+    B = rng.standard_normal((N, K) if bt else (K, N))
+    C0 = rng.standard_normal((M, N))
+    ref = -1.0 * (A.T if at else A) @ (B.T if bt else B) + 1.0 * C0
+    C = T(C0)
+    gemm(T(A), T(B), C, M, N, K, A.shape[1], B.shape[1], N, a_trans=at, b_trans=bt, alpha=-1.0, beta=1.0, tile=64)
+    np.testing.assert_allclose(C.cpu().numpy(), ref, rtol=1e-12, atol=1e-10)
+
+
+def test_gram_narrow_splitk(rng):
+    from dftbaby_amd.engine import gram, Workspace
+    for P, Q in ((300, 40), (40, 40), (700, 20)):
+        X = rng.standard_normal((P, 50021))
+        Y = rng.standard_normal((Q, 50021))
+        out = torch.zeros((P, Q), dtype=torch.float64, device=_dev())
+        gram(T(X), P, T(Y), Q, 50021, out, Workspace(_dev()))
+        np.testing.assert_allclose(out.cpu().numpy(), X @ Y.T, rtol=1e-11, atol=1e-9)
+
+
 def test_gemm_f64_batched_remap_splitk(rng):
     from dftbaby_amd.engine import gemm
     from dftbaby_amd import _lib
