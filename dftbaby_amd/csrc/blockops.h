@@ -43,7 +43,11 @@ __device__ double blk_sum(double v, double* sh) {
 }
 
 // Sturm count: number of eigenvalues of the tridiagonal (d, e) below x.
-__device__ int sturm_count(const double* d, const double* e, int l, double x, double pivmin) {
+typedef __attribute__((address_space(3))) double lds_f64;
+typedef __attribute__((address_space(3))) int lds_i32;
+
+// d, e in LDS (address-space-3 pointers: ds_read instead of flat loads)
+__device__ int sturm_count(const lds_f64* d, const lds_f64* e, int l, double x, double pivmin) {
   int c = 0;
   double q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
@@ -94,61 +98,62 @@ __device__ void tri_factor(const double* __restrict__ d, const double* __restric
 
 // x <- (T - lam I)^{-1} x with the factors of tri_factor.  The entries being
 // eliminated are carried in registers; the operands of TRI_B consecutive steps
-// are loaded together before their dependent chain, so one memory latency is
-// paid per block of steps instead of per step.
+// are loaded together, without bounds tests (full blocks first, then the
+// remainder one step at a time), so one memory latency is paid per block of
+// steps instead of per step.
 #define TRI_B 8
-__device__ void tri_apply(int l, const double* __restrict__ u0r, const double* __restrict__ u1,
-                          const double* __restrict__ u2, const double* __restrict__ mlt,
-                          const int* __restrict__ piv, double* __restrict__ x) {
+template <typename D, typename I>
+__device__ void tri_apply(int l, const D* __restrict__ u0r, const D* __restrict__ u1, const D* __restrict__ u2,
+                          const D* __restrict__ mlt, const I* __restrict__ piv, D* __restrict__ x) {
   double c = x[0];
-  for (int kb = 0; kb < l - 1; kb += TRI_B) {
+  int k = 0;
+  for (; k + TRI_B <= l - 1; k += TRI_B) {
     double nx[TRI_B], m[TRI_B];
     int pv[TRI_B];
 #pragma unroll
     for (int u = 0; u < TRI_B; ++u) {
-      const int k = kb + u;
-      const bool in = k < l - 1;
-      nx[u] = in ? x[k + 1] : 0.0;
-      m[u] = in ? mlt[k] : 0.0;
-      pv[u] = in ? piv[k] : 0;
+      nx[u] = x[k + u + 1];
+      m[u] = mlt[k + u];
+      pv[u] = piv[k + u];
     }
 #pragma unroll
     for (int u = 0; u < TRI_B; ++u) {
-      const int k = kb + u;
-      if (k < l - 1) {
-        if (pv[u]) {
-          x[k] = nx[u];
-          c = c - m[u] * nx[u];
-        } else {
-          x[k] = c;
-          c = nx[u] - m[u] * c;
-        }
-      }
+      const double keep = pv[u] ? nx[u] : c;
+      c = pv[u] ? c - m[u] * nx[u] : nx[u] - m[u] * c;
+      x[k + u] = keep;
     }
+  }
+  for (; k < l - 1; ++k) {
+    const double nx = x[k + 1], m = mlt[k];
+    const double keep = piv[k] ? nx : c;
+    c = piv[k] ? c - m * nx : nx - m * c;
+    x[k] = keep;
   }
   x[l - 1] = c;
   double x1 = 0.0, x2 = 0.0;
-  for (int kb = l - 1; kb >= 0; kb -= TRI_B) {
+  k = l - 1;
+  for (; k - TRI_B + 1 >= 0; k -= TRI_B) {
     double xv[TRI_B], a1[TRI_B], a2[TRI_B], r0[TRI_B];
 #pragma unroll
     for (int u = 0; u < TRI_B; ++u) {
-      const int k = kb - u;
-      const bool in = k >= 0;
-      xv[u] = in ? x[k] : 0.0;
-      a1[u] = in ? u1[k] : 0.0;
-      a2[u] = in ? u2[k] : 0.0;
-      r0[u] = in ? u0r[k] : 0.0;
+      xv[u] = x[k - u];
+      a1[u] = u1[k - u];
+      a2[u] = u2[k - u];
+      r0[u] = u0r[k - u];
     }
 #pragma unroll
     for (int u = 0; u < TRI_B; ++u) {
-      const int k = kb - u;
-      if (k >= 0) {
-        const double sv = (xv[u] - a1[u] * x1 - a2[u] * x2) * r0[u];
-        x[k] = sv;
-        x2 = x1;
-        x1 = sv;
-      }
+      const double sv = (xv[u] - a1[u] * x1 - a2[u] * x2) * r0[u];
+      x[k - u] = sv;
+      x2 = x1;
+      x1 = sv;
     }
+  }
+  for (; k >= 0; --k) {
+    const double sv = (x[k] - u1[k] * x1 - u2[k] * x2) * u0r[k];
+    x[k] = sv;
+    x2 = x1;
+    x1 = sv;
   }
 }
 
@@ -268,6 +273,8 @@ __device__ __noinline__ void syevx_tridiag_vectors(int l, int nev, double* __res
   __syncthreads();
   const double* dl = sv;
   const double* el = sp;
+  const lds_f64* ldl = (const lds_f64*)sv;   // the same LDS buffers, address-space typed
+  const lds_f64* lel = (const lds_f64*)sp;
   const double lo0 = s_bnd[0], hi0 = s_bnd[1];
   const double tnorm = fmax(fabs(lo0), fabs(hi0));
   double emax2 = 0.0;
@@ -286,7 +293,7 @@ __device__ __noinline__ void syevx_tridiag_vectors(int l, int nev, double* __res
       double a = lo0 - 2.0 * eps * tnorm - pivmin, b = hi0 + 2.0 * eps * tnorm + pivmin;
       for (int it = 0; it < 200; ++it) {
         double mid = 0.5 * (a + b);
-        if (sturm_count(dl, el, l, mid, pivmin) > m) b = mid; else a = mid;
+        if (sturm_count(ldl, lel, l, mid, pivmin) > m) b = mid; else a = mid;
         if (b - a <= 2.0 * eps * fmax(fabs(a), fabs(b)) + pivmin) break;
       }
       evals[m] = 0.5 * (a + b);
@@ -307,7 +314,7 @@ __device__ __noinline__ void syevx_tridiag_vectors(int l, int nev, double* __res
         if (!flag[m]) {
           const double a = ivl[2 * m], b = ivl[2 * m + 1];
           const double x = a + (b - a) * (double)(k + 1) / (double)(P + 1);
-          cnt[m * P + k] = sturm_count(dl, el, l, x, pivmin);
+          cnt[m * P + k] = sturm_count(ldl, lel, l, x, pivmin);
         }
       }
       __syncthreads();
@@ -413,7 +420,13 @@ __device__ __noinline__ void syevx_tridiag_vectors(int l, int nev, double* __res
         for (int i = tid; i < l; i += blockDim.x) x[i] = hash_unit((unsigned)m, (unsigned)i);
         __syncthreads();
         for (int it = 0; it < 3; ++it) {
-          if (tid == 0) tri_apply(l, fu0, fu1, fu2, fml, fpv, x);
+          if (tid == 0) {
+            if (stage)
+              tri_apply(l, (const lds_f64*)fu0, (const lds_f64*)fu1, (const lds_f64*)fu2, (const lds_f64*)fml,
+                        (const lds_i32*)fpv, (lds_f64*)x);
+            else
+              tri_apply(l, fu0, fu1, fu2, fml, fpv, x);
+          }
           __syncthreads();
           for (int pass = 0; pass < 2 && m > m0; ++pass) {
             for (int mm = m0 + wv; mm < m; mm += nw) {
