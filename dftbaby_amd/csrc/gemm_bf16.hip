@@ -191,7 +191,6 @@ __global__ __launch_bounds__(256) void gemm_bf3_kernel(GemmArgs p) {
 // LDS arrays so the compiler's own waits for the DMA stay precise; the
 // cross-wave hand-off is an explicit vmcnt wait followed by s_barrier.
 #define P_BM 256
-#define P_BN 128
 #define P_BK 32
 
 typedef int i4_t __attribute__((ext_vector_type(4)));
@@ -223,32 +222,38 @@ __device__ __forceinline__ void dma16(i4_t rs, unsigned lds, unsigned off) {
 #define P_WAIT_BARRIER(N) asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory")
 
 // NW wave64s per workgroup: NW / 2 along M, 2 along N; wave tile
-// (256 / (NW / 2)) x 64 computed with v_mfma_f32_32x32x16_bf16 (M32; that
-// instruction sustains about twice the rate of 16x16x32 on gfx950) or with
-// v_mfma_f32_16x16x32_bf16.
-template <int NW, bool M32>
+// (256 / (NW / 2)) x (BN / 2) computed with v_mfma_f32_32x32x16_bf16 (M32) or
+// with v_mfma_f32_16x16x32_bf16.  BN = 128: three-stage ring of 48 KiB stages;
+// BN = 256: two-stage ring of 64 KiB stages (the ring must fit the 160 KiB of
+// LDS), 1.5x the MFMA work per byte staged into LDS.
+template <int NW, bool M32, int BN>
 __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
+  static_assert(BN == 128 || BN == 256, "pipelined tile width");
+  constexpr int NST = BN == 128 ? 3 : 2;     // ring stages
   constexpr int WMW = NW / 2;                // waves along M
   constexpr int WROWS = P_BM / WMW;          // rows per wave (64 or 128)
+  constexpr int WCOLS = BN / 2;              // columns per wave (64 or 128)
   constexpr int TM = WROWS / 16;             // 16x16 tiles per wave along M
-  constexpr int TM32 = WROWS / 32;           // 32x32 tiles per wave along M (two along N)
-  constexpr int CA = 16 / NW, CB = 8 / NW;   // 1 KB DMA chunks per wave per A / B array
+  constexpr int TN = WCOLS / 16;             // 16x16 tiles per wave along N
+  constexpr int TM32 = WROWS / 32;           // 32x32 tiles per wave along M
+  constexpr int TN32 = WCOLS / 32;           // 32x32 tiles per wave along N
+  constexpr int CA = 16 / NW, CB = (BN / 16) / NW;  // 1 KB DMA chunks per wave per A / B array
   constexpr int LOADS = 2 * CA + 2 * CB;     // DMA instructions per wave per stage
-  // ring of three stages, each [A hi | A lo | B hi | B lo] (48 KiB)
-  constexpr int SA = P_BM * P_BK, SB = P_BN * P_BK, STAGE = 2 * SA + 2 * SB;
-  __shared__ __attribute__((aligned(1024))) u16 ring[3 * STAGE];
+  // ring of NST stages, each [A hi | A lo | B hi | B lo]
+  constexpr int SA = P_BM * P_BK, SB = BN * P_BK, STAGE = 2 * SA + 2 * SB;
+  __shared__ __attribute__((aligned(1024))) u16 ring[NST * STAGE];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: the LDS destination goes to M0
   const int wm = wave >> 1, wn = wave & 1;
   unsigned tx, ty, tz;
-  xcd_tile_grouped<4>(tx, ty, tz);   // 4 x 8 tiles (1024 x 1024) per XCD at a time
-  const long n0 = (long)tx * P_BN, m0 = (long)ty * P_BM;
+  xcd_tile_grouped<4>(tx, ty, tz);   // 4 x 8 tiles (1024 x 8 BN) per XCD at a time
+  const long n0 = (long)tx * BN, m0 = (long)ty * P_BM;
   const int ks = tz % p.ksplit;
   const long b = tz / p.ksplit;
   const long kbeg = (long)ks * p.kchunk;
   const long kend = min((long)p.K, kbeg + p.kchunk);
   const long M = p.M, N = p.N;
-  const long mrows = min((long)P_BM, M - m0), nrows = min((long)P_BN, N - n0);
+  const long mrows = min((long)P_BM, M - m0), nrows = min((long)BN, N - n0);
 
   // buffer descriptors over this workgroup's rows; an offset equal to num_records loads zeros
   const unsigned nbA = (unsigned)(((mrows - 1) * p.lda + kend) * 2);
@@ -305,22 +310,26 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
 
   const int nk = (kend > kbeg) ? (int)((kend - kbeg + P_BK - 1) / P_BK) : 0;
 #define P_SLOT(st) (ring + (st) * STAGE)
+  // Stages 0 .. NST - 2 are issued up front.  At the top of step kt the loads of
+  // stages kt .. min(kt + NST - 2, nk - 1) are in flight; stage kt is complete once
+  // at most the later ones remain (vmcnt counts this wave's loads in issue order),
+  // and the barrier also tells every wave that all reads of step kt - 1 are done,
+  // so the slot of stage kt + NST - 1 (= that of kt - 1) may be refilled.
 #define P_MAINLOOP(COMPUTE)                                                           \
   if (nk > 0) {                                                                       \
     { u16* S0 = P_SLOT(0); P_ISSUE(0, S0, S0 + SA, S0 + 2 * SA, S0 + 2 * SA + SB); }  \
-    if (nk > 1) { u16* S1 = P_SLOT(1); P_ISSUE(1, S1, S1 + SA, S1 + 2 * SA, S1 + 2 * SA + SB); } \
+    if (NST == 3 && nk > 1) { u16* S1 = P_SLOT(1); P_ISSUE(1, S1, S1 + SA, S1 + 2 * SA, S1 + 2 * SA + SB); } \
     int cur = 0;                                                                      \
     for (int kt = 0; kt < nk; ++kt) {                                                 \
-      /* stage kt is complete once at most the next stage's loads remain in flight */ \
-      if (kt + 1 < nk) P_WAIT_BARRIER(LOADS); else P_WAIT_BARRIER(0);                 \
-      if (kt + 2 < nk) {                                                              \
-        const int nx = cur == 0 ? 2 : cur - 1; /* slot of stage kt + 2 */             \
+      if (NST == 3 && kt + 1 < nk) P_WAIT_BARRIER(LOADS); else P_WAIT_BARRIER(0);     \
+      if (kt + NST - 1 < nk) {                                                        \
+        const int nx = cur == 0 ? NST - 1 : cur - 1; /* slot of stage kt + NST - 1 */ \
         u16* S = P_SLOT(nx);                                                          \
-        P_ISSUE(kt + 2, S, S + SA, S + 2 * SA, S + 2 * SA + SB);                      \
+        P_ISSUE(kt + NST - 1, S, S + SA, S + 2 * SA, S + 2 * SA + SB);                \
       }                                                                               \
       const u16* C = P_SLOT(cur);                                                     \
       COMPUTE(C, C + SA, C + 2 * SA, C + 2 * SA + SB);                                \
-      cur = cur == 2 ? 0 : cur + 1;                                                   \
+      cur = cur == NST - 1 ? 0 : cur + 1;                                             \
     }                                                                                 \
   }
   const long cbase = b * p.sC + (long)ks * p.sCk;
@@ -331,25 +340,25 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
     // fragment of a 32x32x16 MFMA: lane l reads row l % 32 at k-chunk 2 s + l / 32
     // (k-step s of the stage), stored in slot chunk ^ ((row >> 2) & 3)
     typedef float f16v_t __attribute__((ext_vector_type(16)));
-    f16v_t acc[TM32][2];
+    f16v_t acc[TM32][TN32];
 #pragma unroll
     for (int i = 0; i < TM32; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = (f16v_t)(0.0f);
+      for (int j = 0; j < TN32; ++j) acc[i][j] = (f16v_t)(0.0f);
     const int sw = (lane >> 2) & 3;
     const int sl0 = ((lane >> 5) ^ sw) * 8, sl1 = ((2 + (lane >> 5)) ^ sw) * 8;
     const int arow = (wm * WROWS + (lane & 31)) * P_BK;
-    const int brow = (wn * 64 + (lane & 31)) * P_BK;
+    const int brow = (wn * WCOLS + (lane & 31)) * P_BK;
 #define P_COMPUTE32(Ah, Al, Bh, Bl)                                                              \
   do {                                                                                           \
-    bf8_t ah_[TM32][2], al_[TM32][2], bh_[2][2], bl_[2][2];                                      \
+    bf8_t ah_[TM32][2], al_[TM32][2], bh_[TN32][2], bl_[TN32][2];                                      \
     _Pragma("unroll") for (int i = 0; i < TM32; ++i) {                                           \
       ah_[i][0] = *(const bf8_t*)((Ah) + arow + i * 32 * P_BK + sl0);                            \
       ah_[i][1] = *(const bf8_t*)((Ah) + arow + i * 32 * P_BK + sl1);                            \
       al_[i][0] = *(const bf8_t*)((Al) + arow + i * 32 * P_BK + sl0);                            \
       al_[i][1] = *(const bf8_t*)((Al) + arow + i * 32 * P_BK + sl1);                            \
     }                                                                                            \
-    _Pragma("unroll") for (int j = 0; j < 2; ++j) {                                              \
+    _Pragma("unroll") for (int j = 0; j < TN32; ++j) {                                              \
       bh_[j][0] = *(const bf8_t*)((Bh) + brow + j * 32 * P_BK + sl0);                            \
       bh_[j][1] = *(const bf8_t*)((Bh) + brow + j * 32 * P_BK + sl1);                            \
       bl_[j][0] = *(const bf8_t*)((Bl) + brow + j * 32 * P_BK + sl0);                            \
@@ -357,11 +366,11 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
     }                                                                                            \
     /* product-major order: independent MFMAs separate two updates of one accumulator */        \
     _Pragma("unroll") for (int s_ = 0; s_ < 2; ++s_) {                                           \
-      _Pragma("unroll") for (int i = 0; i < TM32; ++i) _Pragma("unroll") for (int j = 0; j < 2; ++j) \
+      _Pragma("unroll") for (int i = 0; i < TM32; ++i) _Pragma("unroll") for (int j = 0; j < TN32; ++j) \
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al_[i][s_], bh_[j][s_], acc[i][j], 0, 0, 0); \
-      _Pragma("unroll") for (int i = 0; i < TM32; ++i) _Pragma("unroll") for (int j = 0; j < 2; ++j) \
+      _Pragma("unroll") for (int i = 0; i < TM32; ++i) _Pragma("unroll") for (int j = 0; j < TN32; ++j) \
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah_[i][s_], bl_[j][s_], acc[i][j], 0, 0, 0); \
-      _Pragma("unroll") for (int i = 0; i < TM32; ++i) _Pragma("unroll") for (int j = 0; j < 2; ++j) \
+      _Pragma("unroll") for (int i = 0; i < TM32; ++i) _Pragma("unroll") for (int j = 0; j < TN32; ++j) \
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah_[i][s_], bh_[j][s_], acc[i][j], 0, 0, 0); \
     }                                                                                            \
   } while (0)
@@ -370,28 +379,28 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
     // epilogue: accumulator entry q of tile (i, j) is row 8 (q / 4) + 4 (l / 32) + q % 4,
     // column l % 32.  Column parts of the offsets and of the Hadamard index are formed
     // once; each tile row block loads its Hadamard factors before its stores.
-    long coff[2], hcol[2];
+    long coff[TN32], hcol[TN32];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const long col = n0 + wn * 64 + j * 32 + (lane & 31);
+    for (int j = 0; j < TN32; ++j) {
+      const long col = n0 + wn * WCOLS + j * 32 + (lane & 31);
       coff[j] = gemm_col_part(p, col);
       hcol[j] = had_on ? gemm_had_col(p, col) : 0;
     }
 #define P_EPI32(STMT)                                                                  \
   _Pragma("unroll") for (int i = 0; i < TM32; ++i) {                                   \
-    float hv_[16][2];                                                                  \
+    float hv_[16][TN32];                                                                  \
     _Pragma("unroll") for (int q = 0; q < 16; ++q) {                                   \
       const long row = m0 + wm * WROWS + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3); \
       const long hr = had_on ? gemm_had_row(p, row) : 0;                               \
-      _Pragma("unroll") for (int j = 0; j < 2; ++j)                                    \
+      _Pragma("unroll") for (int j = 0; j < TN32; ++j)                                    \
         hv_[q][j] = had_on ? (float)p.had[hr + hcol[j]] : 1.0f;                        \
     }                                                                                  \
     _Pragma("unroll") for (int q = 0; q < 16; ++q) {                                   \
       const long row = m0 + wm * WROWS + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3); \
       if (row < M) {                                                                   \
         const long ro = cbase + gemm_row_part(p, row);                                 \
-        _Pragma("unroll") for (int j = 0; j < 2; ++j) {                                \
-          const long col = n0 + wn * 64 + j * 32 + (lane & 31);                        \
+        _Pragma("unroll") for (int j = 0; j < TN32; ++j) {                                \
+          const long col = n0 + wn * WCOLS + j * 32 + (lane & 31);                        \
           if (col < N) {                                                               \
             const long off = ro + coff[j];                                             \
             const float val = alpha * hv_[q][j] * acc[i][j][q];                        \
@@ -420,62 +429,67 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
     }
 #undef P_EPI32
   } else {
-    f4_t acc[TM][4];
+    f4_t acc[TM][TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = (f4_t)(0.0f);
+      for (int j = 0; j < TN; ++j) acc[i][j] = (f4_t)(0.0f);
     // fragment reads: row r = 16 q + (lane & 15) holds k-chunk c = lane >> 4 in slot c ^ ((lane >> 2) & 3)
     const int slot = ((lane >> 4) ^ ((lane >> 2) & 3)) * 8;
     const int arow = (wm * WROWS + (lane & 15)) * P_BK + slot;
-    const int brow = (wn * 64 + (lane & 15)) * P_BK + slot;
+    const int brow = (wn * WCOLS + (lane & 15)) * P_BK + slot;
+    // B fragments in groups of four columns of 16 (one group per 64 columns) so the
+    // 256-wide tile's fragments and accumulators fit the 256 registers of a wave
 #define P_COMPUTE(Ah, Al, Bh, Bl)                                                                    \
   do {                                                                                               \
-    bf8_t ah_[TM], al_[TM], bh_[4], bl_[4];                                                          \
+    bf8_t ah_[TM], al_[TM];                                                                          \
     _Pragma("unroll") for (int i = 0; i < TM; ++i) {                                                 \
       ah_[i] = *(const bf8_t*)((Ah) + arow + i * 16 * P_BK);                                         \
       al_[i] = *(const bf8_t*)((Al) + arow + i * 16 * P_BK);                                         \
     }                                                                                                \
-    _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                                  \
-      bh_[j] = *(const bf8_t*)((Bh) + brow + j * 16 * P_BK);                                         \
-      bl_[j] = *(const bf8_t*)((Bl) + brow + j * 16 * P_BK);                                         \
+    _Pragma("unroll") for (int g_ = 0; g_ < TN / 4; ++g_) {                                          \
+      bf8_t bh_[4], bl_[4];                                                                          \
+      _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                                \
+        bh_[j] = *(const bf8_t*)((Bh) + brow + (4 * g_ + j) * 16 * P_BK);                            \
+        bl_[j] = *(const bf8_t*)((Bl) + brow + (4 * g_ + j) * 16 * P_BK);                            \
+      }                                                                                              \
+      _Pragma("unroll") for (int i = 0; i < TM; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j)   \
+        acc[i][4 * g_ + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al_[i], bh_[j], acc[i][4 * g_ + j], 0, 0, 0); \
+      _Pragma("unroll") for (int i = 0; i < TM; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j)   \
+        acc[i][4 * g_ + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah_[i], bl_[j], acc[i][4 * g_ + j], 0, 0, 0); \
+      _Pragma("unroll") for (int i = 0; i < TM; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j)   \
+        acc[i][4 * g_ + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah_[i], bh_[j], acc[i][4 * g_ + j], 0, 0, 0); \
     }                                                                                                \
-    _Pragma("unroll") for (int i = 0; i < TM; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j)     \
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al_[i], bh_[j], acc[i][j], 0, 0, 0);       \
-    _Pragma("unroll") for (int i = 0; i < TM; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j)     \
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah_[i], bl_[j], acc[i][j], 0, 0, 0);       \
-    _Pragma("unroll") for (int i = 0; i < TM; ++i) _Pragma("unroll") for (int j = 0; j < 4; ++j)     \
-      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah_[i], bh_[j], acc[i][j], 0, 0, 0);       \
   } while (0)
     P_MAINLOOP(P_COMPUTE)
 #undef P_COMPUTE
-    long roff[TM][4], hrow[TM][4], coff[4], hcol[4];
+    // column parts once; row parts per tile row (hoisting all of them would hold
+    // 4 TM 64-bit offsets beside the accumulators)
+    long coff[TN], hcol[TN];
 #pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const long row = m0 + wm * WROWS + i * 16 + 4 * (lane >> 4) + r;
-        roff[i][r] = cbase + gemm_row_part(p, row);
-        hrow[i][r] = had_on ? gemm_had_row(p, row) : 0;
-      }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const long col = n0 + wn * 64 + j * 16 + (lane & 15);
+    for (int j = 0; j < TN; ++j) {
+      const long col = n0 + wn * WCOLS + j * 16 + (lane & 15);
       coff[j] = gemm_col_part(p, col);
       hcol[j] = had_on ? gemm_had_col(p, col) : 0;
     }
 #define P_EPI(STMT)                                                            \
   _Pragma("unroll") for (int i = 0; i < TM; ++i) {                             \
-    float hv_[4][4];                                                           \
-    _Pragma("unroll") for (int j = 0; j < 4; ++j)                              \
-    _Pragma("unroll") for (int r = 0; r < 4; ++r)                              \
-      hv_[j][r] = had_on ? (float)p.had[hrow[i][r] + hcol[j]] : 1.0f;          \
-    _Pragma("unroll") for (int j = 0; j < 4; ++j)                              \
+    long roff_[4], hrow_[4];                                                   \
     _Pragma("unroll") for (int r = 0; r < 4; ++r) {                            \
       const long row = m0 + wm * WROWS + i * 16 + 4 * (lane >> 4) + r;         \
-      const long col = n0 + wn * 64 + j * 16 + (lane & 15);                    \
+      roff_[r] = cbase + gemm_row_part(p, row);                                \
+      hrow_[r] = had_on ? gemm_had_row(p, row) : 0;                            \
+    }                                                                          \
+    float hv_[TN][4];                                                          \
+    _Pragma("unroll") for (int j = 0; j < TN; ++j)                             \
+    _Pragma("unroll") for (int r = 0; r < 4; ++r)                              \
+      hv_[j][r] = had_on ? (float)p.had[hrow_[r] + hcol[j]] : 1.0f;            \
+    _Pragma("unroll") for (int j = 0; j < TN; ++j)                             \
+    _Pragma("unroll") for (int r = 0; r < 4; ++r) {                            \
+      const long row = m0 + wm * WROWS + i * 16 + 4 * (lane >> 4) + r;         \
+      const long col = n0 + wn * WCOLS + j * 16 + (lane & 15);                 \
       if (row < M && col < N) {                                                \
-        const long off = roff[i][r] + coff[j];                                 \
+        const long off = roff_[r] + coff[j];                                   \
         const float val = alpha * hv_[j][r] * acc[i][j][r];                    \
         STMT;                                                                  \
       }                                                                        \
@@ -520,21 +534,35 @@ int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st) {
   if (p.ksplit > 1 && (p.kchunk & 7)) return DFTB_ERR_ARG;
   if ((((uintptr_t)p.A) | ((uintptr_t)p.A2) | ((uintptr_t)p.B) | ((uintptr_t)p.B2)) & 15) return DFTB_ERR_ARG;
   const bool small = p.tile == 64 || (p.tile == 0 && (p.M < 128 || p.N < 128));
-  const bool pipe = p.tile == 256 || (p.tile == 0 && p.M >= 256 && p.N >= 128 && p.K >= 4 * P_BK);
+  const bool pipe = p.tile == 256 || p.tile == 512 ||
+                    (p.tile == 0 && p.M >= 256 && p.N >= 128 && p.K >= 4 * P_BK);
   if (pipe) {
+    // tile 256: 256 x 128 tiles; tile 512: 256 x 256 tiles; automatic: 256 x 256
+    // when N >= 256 unless DFTB_BF3_BN=128 asks for the narrower tile
+    static const int bn_env = getenv("DFTB_BF3_BN") ? atoi(getenv("DFTB_BF3_BN")) : 256;
+    const int bn = p.tile == 512 ? 256 : p.tile == 256 ? 128 : (bn_env == 128 || p.N < 256) ? 128 : 256;
     // 32-bit buffer offsets: each workgroup's operand span must stay below 4 GiB
-    const long span_a = ((long)P_BM * p.lda + p.K) * 2, span_b = ((long)P_BN * p.ldb + p.K) * 2;
+    const long span_a = ((long)P_BM * p.lda + p.K) * 2, span_b = ((long)bn * p.ldb + p.K) * 2;
     if (span_a >= 0xffff0000L || span_b >= 0xffff0000L) return DFTB_ERR_ARG;
-    dim3 grid(ceil_div(p.N, P_BN), ceil_div(p.M, P_BM), p.batch * p.ksplit);
+    dim3 grid(ceil_div(p.N, bn), ceil_div(p.M, P_BM), p.batch * p.ksplit);
     GemmTimer tm;
     tm.begin(st);
     // 16x16x32 by default: the kernel is bound by its global->LDS feed, where the
-    // 16x16 form measured 2-4% faster; DFTB_BF3_M32=1 selects the 32x32x16 form
+    // 16x16 form measured 2-4% faster (256 x 128 tiles); DFTB_BF3_M32=1 selects the
+    // 32x32x16 form
     static const int m32 = getenv("DFTB_BF3_M32") ? atoi(getenv("DFTB_BF3_M32")) : 0;
-    if (m32)
-      hipLaunchKernelGGL((gemm_bf3_pipe_kernel<DFTB_BF3_WAVES, true>), grid, dim3(DFTB_BF3_WAVES * 64), 0, st, p);
-    else
-      hipLaunchKernelGGL((gemm_bf3_pipe_kernel<DFTB_BF3_WAVES, false>), grid, dim3(DFTB_BF3_WAVES * 64), 0, st, p);
+    const dim3 block(DFTB_BF3_WAVES * 64);
+    if (bn == 256) {
+      if (m32)
+        hipLaunchKernelGGL((gemm_bf3_pipe_kernel<DFTB_BF3_WAVES, true, 256>), grid, block, 0, st, p);
+      else
+        hipLaunchKernelGGL((gemm_bf3_pipe_kernel<DFTB_BF3_WAVES, false, 256>), grid, block, 0, st, p);
+    } else {
+      if (m32)
+        hipLaunchKernelGGL((gemm_bf3_pipe_kernel<DFTB_BF3_WAVES, true, 128>), grid, block, 0, st, p);
+      else
+        hipLaunchKernelGGL((gemm_bf3_pipe_kernel<DFTB_BF3_WAVES, false, 128>), grid, block, 0, st, p);
+    }
     tm.end(st, GT_BF3_PIPE, 3.0 * 2.0 * p.M * p.N * (double)p.K * p.batch);
     DFTB_CHECK_LAUNCH();
     return DFTB_OK;

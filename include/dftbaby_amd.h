@@ -59,7 +59,7 @@ typedef struct GemmArgs {
   double alpha, beta;
   int a_trans, b_trans;
   int in_type, out_type;
-  int tile; /* 0 auto, 64 or 128 */
+  int tile; /* 0 auto, 64 or 128; BF16x3 only: 256 = pipelined 256x128, 512 = pipelined 256x256 */
 } GemmArgs;
 
 int dftb_gemm(const GemmArgs* p, void* stream);
@@ -71,7 +71,7 @@ int dftb_gemm(const GemmArgs* p, void* stream);
  * class cls and returns their summed device time (ms), count and algorithmic
  * flops (2 M N K per batch entry, times 3 for the BF16x3 split product).
  * Classes: 0 FP64 128x128 (no transposed operand), 1 FP64 64x64, 2 FP32
- * 128x128, 3 FP32 64x64, 4 BF16x3 pipelined 256x128, 5 BF16x3 64/128 tiles,
+ * 128x128, 3 FP32 64x64, 4 BF16x3 pipelined (256x128 or 256x256), 5 BF16x3 64/128 tiles,
  * 6 FP64 128x128 with a transposed operand. */
 enum { GEMM_CLASS_F64_128 = 0, GEMM_CLASS_F64_64 = 1, GEMM_CLASS_F32_128 = 2, GEMM_CLASS_F32_64 = 3,
        GEMM_CLASS_BF16X3_PIPE = 4, GEMM_CLASS_BF16X3_TILE = 5, GEMM_CLASS_F64_128_T = 6 };

@@ -151,7 +151,9 @@ def _bf_pair(X):
 
 
 @pytest.mark.parametrize("M,N,K,batch,tile", [(256, 128, 128, 1, 256), (300, 260, 200, 2, 256), (513, 300, 1000, 3, 0),
-                                              (260, 129, 136, 1, 0)])
+                                              (260, 129, 136, 1, 0), (256, 256, 128, 1, 512),
+                                              (300, 520, 200, 2, 512), (513, 300, 1000, 3, 512),
+                                              (260, 129, 136, 1, 512)])
 def test_gemm_bf16x3_pipelined(M, N, K, batch, tile, rng):
     """Large-tile pipelined BF16x3 kernel: partial tiles in M, N and K (K tail not a
     multiple of 32 loads zeros), strided batches, beta accumulation."""
@@ -179,7 +181,8 @@ def test_gemm_bf16x3_pipelined(M, N, K, batch, tile, rng):
     assert err < 3e-5 * np.sqrt(K), err
 
 
-def test_gemm_bf16x3_pipelined_remap_pair_output(rng):
+@pytest.mark.parametrize("tile", [256, 512])
+def test_gemm_bf16x3_pipelined_remap_pair_output(tile, rng):
     """Column-remapped bf16-pair output (the exchange 'expand' form) of the pipelined kernel."""
     from dftbaby_amd import _lib
     from dftbaby_amd.engine import _stream
@@ -198,7 +201,7 @@ def test_gemm_bf16x3_pipelined_remap_pair_output(rng):
     g.M, g.N, g.K, g.batch, g.ksplit = M, N, K, 1, 1
     g.alpha, g.beta = 1.0, 0.0
     g.b_trans = 1
-    g.tile = 256
+    g.tile = tile
     g.in_type, g.out_type = _lib.GEMM_IN_BF16X3, _lib.GEMM_OUT_BF16PAIR
     _lib.call("dftb_gemm", ctypes.byref(g), _stream())
     f = lambda t: (t.to(torch.int32) & 0xFFFF).cpu().numpy().astype(np.uint32) << 16
@@ -210,7 +213,8 @@ def test_gemm_bf16x3_pipelined_remap_pair_output(rng):
     assert (np.abs(got - ref) <= 3e-5 * np.sqrt(K) + 2.0 ** -15 * np.abs(ref)).all()
 
 
-def test_gemm_bf16x3_pipelined_row_remap(rng):
+@pytest.mark.parametrize("tile", [256, 512])
+def test_gemm_bf16x3_pipelined_row_remap(tile, rng):
     """Row-remapped bf16-pair output (the swapped exchange 'expand'): row (g, a) -> a * rs2 + g * rstr."""
     from dftbaby_amd import _lib
     from dftbaby_amd.engine import _stream
@@ -230,7 +234,7 @@ def test_gemm_bf16x3_pipelined_row_remap(rng):
     g.M, g.N, g.K, g.batch, g.ksplit = M, N, K, 1, 1
     g.alpha, g.beta = 1.0, 0.0
     g.b_trans = 1
-    g.tile = 256
+    g.tile = tile
     g.in_type, g.out_type = _lib.GEMM_IN_BF16X3, _lib.GEMM_OUT_BF16PAIR
     _lib.call("dftb_gemm", ctypes.byref(g), _stream())
     f = lambda t: (t.to(torch.int32) & 0xFFFF).cpu().numpy().astype(np.uint32) << 16
