@@ -221,6 +221,65 @@ __device__ __forceinline__ void dma16(i4_t rs, unsigned lds, unsigned off) {
 // reads have landed, then barrier
 #define P_WAIT_BARRIER(N) asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory")
 
+// Hadamard factor of one output tile, read from LDS.  The factor of element
+// (row, col) is had[ra(row) ldh + cb(col)] with ra, cb the row / column keys
+// (hmap of row % hmod, or row % hmod itself).  After the main loop the ring is
+// free: the key ranges [a0, a1] x [b0, b1] of the tile's rows and columns are
+// reduced in LDS, and when that block of the factor fits, it is copied there
+// as float once per tile (row stride nc) instead of being gathered from global
+// memory per element.  Otherwise (scattered keys) the epilogue reads global
+// memory as before.  Block-uniform result; call with every thread.
+struct HadTile {
+  const float* s;  // LDS block, or nullptr: read p.had
+  int a0, b0;      // key offsets subtracted before indexing
+  long ld;         // row stride of the block (nc, or p.ldh)
+};
+__device__ __forceinline__ int had_rkey(const GemmArgs& p, long row) {
+  const unsigned rm = (unsigned)row % (unsigned)p.hmod;
+  return p.hmap ? p.hmap[rm] : (int)rm;
+}
+template <int BM, int BN, int NT>
+__device__ HadTile had_stage(const GemmArgs& p, long m0, long n0, u16* ring, int cap_floats) {
+  const int tid = threadIdx.x;
+  int* hb = (int*)ring;
+  float* hs = (float*)ring + 64;
+  __syncthreads();  // every wave is done reading the ring
+  if (tid == 0) { hb[0] = 0x7fffffff; hb[1] = -1; hb[2] = 0x7fffffff; hb[3] = -1; }
+  __syncthreads();
+  int k[4] = {0x7fffffff, -1, 0x7fffffff, -1};  // min / max row key, min / max column key
+  for (int r = tid; r < BM; r += NT)
+    if (m0 + r < p.M) { const int a = had_rkey(p, m0 + r); k[0] = min(k[0], a); k[1] = max(k[1], a); }
+  for (int c = tid; c < BN; c += NT)
+    if (n0 + c < p.N) { const int b = had_rkey(p, n0 + c); k[2] = min(k[2], b); k[3] = max(k[3], b); }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    k[0] = min(k[0], __shfl_xor(k[0], o));
+    k[1] = max(k[1], __shfl_xor(k[1], o));
+    k[2] = min(k[2], __shfl_xor(k[2], o));
+    k[3] = max(k[3], __shfl_xor(k[3], o));
+  }
+  if ((tid & 63) == 0) {
+    atomicMin(&hb[0], k[0]); atomicMax(&hb[1], k[1]);
+    atomicMin(&hb[2], k[2]); atomicMax(&hb[3], k[3]);
+  }
+  __syncthreads();
+  const int a0 = hb[0], b0 = hb[2];
+  const int nr = hb[1] - a0 + 1, nc = hb[3] - b0 + 1;
+  HadTile h;
+  if ((long)nr * nc > (long)(cap_floats - 64)) {
+    h.s = nullptr; h.a0 = 0; h.b0 = 0; h.ld = p.ldh;
+    return h;
+  }
+  const int wave = tid >> 6, lane = tid & 63;
+  for (int r = wave; r < nr; r += NT / 64) {
+    const double* src = p.had + (long)(a0 + r) * p.ldh + b0;
+    for (int c = lane; c < nc; c += 64) hs[r * nc + c] = (float)src[c];
+  }
+  __syncthreads();
+  h.s = hs; h.a0 = a0; h.b0 = b0; h.ld = nc;
+  return h;
+}
+
 // NW wave64s per workgroup: NW / 2 along M, 2 along N; wave tile
 // (256 / (NW / 2)) x (BN / 2) computed with v_mfma_f32_32x32x16_bf16 (M32) or
 // with v_mfma_f32_16x16x32_bf16.  BN = 128: three-stage ring of 48 KiB stages;
@@ -376,6 +435,8 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
   } while (0)
     P_MAINLOOP(P_COMPUTE32)
 #undef P_COMPUTE32
+    const HadTile ht = had_on ? had_stage<P_BM, BN, NW * 64>(p, m0, n0, ring, NST * STAGE / 2)
+                              : HadTile{nullptr, 0, 0, 0};
     // epilogue: accumulator entry q of tile (i, j) is row 8 (q / 4) + 4 (l / 32) + q % 4,
     // column l % 32.  Column parts of the offsets and of the Hadamard index are formed
     // once; each tile row block loads its Hadamard factors before its stores.
@@ -384,16 +445,21 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
     for (int j = 0; j < TN32; ++j) {
       const long col = n0 + wn * WCOLS + j * 32 + (lane & 31);
       coff[j] = gemm_col_part(p, col);
-      hcol[j] = had_on ? gemm_had_col(p, col) : 0;
+      hcol[j] = had_on ? (long)(had_rkey(p, col) - ht.b0) : 0;
     }
 #define P_EPI32(STMT)                                                                  \
   _Pragma("unroll") for (int i = 0; i < TM32; ++i) {                                   \
     float hv_[16][TN32];                                                                  \
     _Pragma("unroll") for (int q = 0; q < 16; ++q) {                                   \
       const long row = m0 + wm * WROWS + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3); \
-      const long hr = had_on ? gemm_had_row(p, row) : 0;                               \
-      _Pragma("unroll") for (int j = 0; j < TN32; ++j)                                    \
-        hv_[q][j] = had_on ? (float)p.had[hr + hcol[j]] : 1.0f;                        \
+      const long hr = had_on ? (long)(had_rkey(p, row) - ht.a0) * ht.ld : 0;           \
+      if (!had_on) {                                                                   \
+        _Pragma("unroll") for (int j = 0; j < TN32; ++j) hv_[q][j] = 1.0f;             \
+      } else if (ht.s) {                                                               \
+        _Pragma("unroll") for (int j = 0; j < TN32; ++j) hv_[q][j] = ht.s[hr + hcol[j]]; \
+      } else {                                                                         \
+        _Pragma("unroll") for (int j = 0; j < TN32; ++j) hv_[q][j] = (float)p.had[hr + hcol[j]]; \
+      }                                                                                \
     }                                                                                  \
     _Pragma("unroll") for (int q = 0; q < 16; ++q) {                                   \
       const long row = m0 + wm * WROWS + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3); \
@@ -463,6 +529,8 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
   } while (0)
     P_MAINLOOP(P_COMPUTE)
 #undef P_COMPUTE
+    const HadTile ht = had_on ? had_stage<P_BM, BN, NW * 64>(p, m0, n0, ring, NST * STAGE / 2)
+                              : HadTile{nullptr, 0, 0, 0};
     // column parts once; row parts per tile row (hoisting all of them would hold
     // 4 TM 64-bit offsets beside the accumulators)
     long coff[TN], hcol[TN];
@@ -470,7 +538,7 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
     for (int j = 0; j < TN; ++j) {
       const long col = n0 + wn * WCOLS + j * 16 + (lane & 15);
       coff[j] = gemm_col_part(p, col);
-      hcol[j] = had_on ? gemm_had_col(p, col) : 0;
+      hcol[j] = had_on ? (long)(had_rkey(p, col) - ht.b0) : 0;
     }
 #define P_EPI(STMT)                                                            \
   _Pragma("unroll") for (int i = 0; i < TM; ++i) {                             \
@@ -478,12 +546,21 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
     _Pragma("unroll") for (int r = 0; r < 4; ++r) {                            \
       const long row = m0 + wm * WROWS + i * 16 + 4 * (lane >> 4) + r;         \
       roff_[r] = cbase + gemm_row_part(p, row);                                \
-      hrow_[r] = had_on ? gemm_had_row(p, row) : 0;                            \
+      hrow_[r] = had_on ? (long)(had_rkey(p, row) - ht.a0) * ht.ld : 0;       \
     }                                                                          \
     float hv_[TN][4];                                                          \
-    _Pragma("unroll") for (int j = 0; j < TN; ++j)                             \
-    _Pragma("unroll") for (int r = 0; r < 4; ++r)                              \
-      hv_[j][r] = had_on ? (float)p.had[hrow_[r] + hcol[j]] : 1.0f;            \
+    if (!had_on) {                                                             \
+      _Pragma("unroll") for (int j = 0; j < TN; ++j)                           \
+      _Pragma("unroll") for (int r = 0; r < 4; ++r) hv_[j][r] = 1.0f;          \
+    } else if (ht.s) {                                                         \
+      _Pragma("unroll") for (int j = 0; j < TN; ++j)                           \
+      _Pragma("unroll") for (int r = 0; r < 4; ++r)                            \
+        hv_[j][r] = ht.s[hrow_[r] + hcol[j]];                                  \
+    } else {                                                                   \
+      _Pragma("unroll") for (int j = 0; j < TN; ++j)                           \
+      _Pragma("unroll") for (int r = 0; r < 4; ++r)                            \
+        hv_[j][r] = (float)p.had[hrow_[r] + hcol[j]];                          \
+    }                                                                          \
     _Pragma("unroll") for (int j = 0; j < TN; ++j)                             \
     _Pragma("unroll") for (int r = 0; r < 4; ++r) {                            \
       const long row = m0 + wm * WROWS + i * 16 + 4 * (lane >> 4) + r;         \

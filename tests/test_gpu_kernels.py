@@ -182,6 +182,54 @@ def test_gemm_bf16x3_pipelined(M, N, K, batch, tile, rng):
 
 
 @pytest.mark.parametrize("tile", [256, 512])
+@pytest.mark.parametrize("keys", ["atoms", "scattered", "identity"])
+def test_gemm_bf16x3_pipelined_hadamard(tile, keys, rng):
+    """Hadamard factor in the epilogue of the pipelined kernel: out[r][c] =
+    alpha had[key(r)][key(c)] (A B^T)[r][c] with key(x) = hmap[x % hmod] (or x % hmod).
+    'atoms' (orbital -> atom map, rows wrapping at hmod) takes the per-tile LDS block;
+    'scattered' keys span more than the LDS holds and take the global-memory reads."""
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _stream
+    K = 160
+    if keys == "atoms":
+        nat, hmod = 61, 200
+        per = np.concatenate([np.full(4, a) for a in range(40)] + [np.arange(40, 61)] + [np.full(19, 61)])
+        hmap = per.astype(np.int32)                 # 160 + 21 + 19 = 200 orbitals, padding -> row nat
+        M = N = 2 * hmod
+    elif keys == "scattered":
+        nat, hmod = 299, 520
+        hmap = rng.integers(0, nat + 1, hmod).astype(np.int32)
+        M, N = 520, 520
+    else:
+        nat, hmod, hmap = 63, 64, None
+        M, N = 300, 264
+    ldh = nat + 1
+    gam = rng.standard_normal((ldh, ldh))
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((N, K))
+    (ah, al), (bh, bl) = _bf_pair(T(A)), _bf_pair(T(B))
+    C = torch.zeros((M, N), dtype=torch.float64, device=_dev())
+    G = T(gam)
+    Hm = torch.from_numpy(hmap).to(_dev()) if hmap is not None else None
+    g = _lib.GemmArgs()
+    g.A, g.A2, g.B, g.B2, g.C = ah.data_ptr(), al.data_ptr(), bh.data_ptr(), bl.data_ptr(), C.data_ptr()
+    g.lda, g.ldb, g.ldc, g.cs2 = K, K, N, 1
+    g.M, g.N, g.K, g.batch, g.ksplit = M, N, K, 1, 1
+    g.alpha, g.beta = 0.75, 0.0
+    g.b_trans = 1
+    g.tile = tile
+    g.had, g.hmod, g.ldh = G.data_ptr(), hmod, ldh
+    g.hmap = Hm.data_ptr() if Hm is not None else 0
+    g.in_type, g.out_type = _lib.GEMM_IN_BF16X3, _lib.GEMM_OUT_F64
+    _lib.call("dftb_gemm", ctypes.byref(g), _stream())
+    key = (lambda x: hmap[x % hmod]) if hmap is not None else (lambda x: x % hmod)
+    fac = gam[key(np.arange(M))][:, key(np.arange(N))].astype(np.float32).astype(np.float64)
+    ref = 0.75 * fac * (A @ B.T)
+    err = np.abs(C.cpu().numpy() - ref).max()
+    assert err < 3e-5 * np.sqrt(K) * np.abs(gam).max(), err
+
+
+@pytest.mark.parametrize("tile", [256, 512])
 def test_gemm_bf16x3_pipelined_remap_pair_output(tile, rng):
     """Column-remapped bf16-pair output (the exchange 'expand' form) of the pipelined kernel."""
     from dftbaby_amd import _lib
