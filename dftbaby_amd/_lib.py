@@ -39,7 +39,8 @@ class GemmArgs(ctypes.Structure):
                 ("alpha", _d), ("beta", _d),
                 ("a_trans", _i), ("b_trans", _i),
                 ("in_type", _i), ("out_type", _i),
-                ("tile", _i)]
+                ("tile", _i),
+                ("Ct", _vp), ("Ct2", _vp), ("ldct", _l), ("sCt", _l)]
 
 
 class CasidaOperator(ctypes.Structure):

@@ -64,5 +64,7 @@ static inline GemmArgs gemm_args_default() {
   p.in_type = GEMM_IN_F64;
   p.out_type = GEMM_OUT_F64;
   p.tile = 0;
+  p.Ct = p.Ct2 = nullptr;
+  p.ldct = p.sCt = 0;
   return p;
 }

@@ -279,6 +279,7 @@ int gemm_launch(const GemmArgs& p0, hipStream_t st) {
   const bool big = (p.M >= 128 && p.N >= 128) || p.tile == 128;
   const bool small = p.tile == 64 || !big;
   if (p.in_type == GEMM_IN_BF16X3) return gemm_bf16x3_launch(p, st);
+  if (p.Ct) return DFTB_ERR_ARG;  // transposed copy: pipelined BF16x3 kernel only
   if (p.in_type != GEMM_IN_F64 && p.in_type != GEMM_IN_F32) return DFTB_ERR_ARG;
   GemmTimer tm;
   tm.begin(st);

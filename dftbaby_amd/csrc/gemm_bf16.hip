@@ -221,6 +221,18 @@ __device__ __forceinline__ void dma16(i4_t rs, unsigned lds, unsigned off) {
 // reads have landed, then barrier
 #define P_WAIT_BARRIER(N) asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory")
 
+// Four consecutive values as bf16 (hi, lo) pairs, one 8-byte store per array.
+__device__ __forceinline__ void store_pair4(u16* hi, u16* lo, long off, const float* v) {
+  unsigned h[4], l[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    h[r] = bf16_bits_rne(v[r]);
+    l[r] = bf16_bits_rne(v[r] - bf16_bits_to_float((u16)h[r]));
+  }
+  *(uint2*)(hi + off) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+  *(uint2*)(lo + off) = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
+}
+
 // Hadamard factor of one output tile, read from LDS.  The factor of element
 // (row, col) is had[ra(row) ldh + cb(col)] with ra, cb the row / column keys
 // (hmap of row % hmod, or row % hmod itself).  After the main loop the ring is
@@ -533,14 +545,15 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
                               : HadTile{nullptr, 0, 0, 0};
     // column parts once; row parts per tile row (hoisting all of them would hold
     // 4 TM 64-bit offsets beside the accumulators)
-    long coff[TN], hcol[TN];
+    long coff[TN];
+    int hcol[TN];
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
       const long col = n0 + wn * WCOLS + j * 16 + (lane & 15);
       coff[j] = gemm_col_part(p, col);
-      hcol[j] = had_on ? (long)(had_rkey(p, col) - ht.b0) : 0;
+      hcol[j] = had_on ? had_rkey(p, col) - ht.b0 : 0;
     }
-#define P_EPI(STMT)                                                            \
+#define P_EPI(STMT, ...)                                                       \
   _Pragma("unroll") for (int i = 0; i < TM; ++i) {                             \
     long roff_[4], hrow_[4];                                                   \
     _Pragma("unroll") for (int r = 0; r < 4; ++r) {                            \
@@ -548,46 +561,52 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
       roff_[r] = cbase + gemm_row_part(p, row);                                \
       hrow_[r] = had_on ? (long)(had_rkey(p, row) - ht.a0) * ht.ld : 0;       \
     }                                                                          \
-    float hv_[TN][4];                                                          \
-    if (!had_on) {                                                             \
-      _Pragma("unroll") for (int j = 0; j < TN; ++j)                           \
-      _Pragma("unroll") for (int r = 0; r < 4; ++r) hv_[j][r] = 1.0f;          \
-    } else if (ht.s) {                                                         \
-      _Pragma("unroll") for (int j = 0; j < TN; ++j)                           \
-      _Pragma("unroll") for (int r = 0; r < 4; ++r)                            \
-        hv_[j][r] = ht.s[hrow_[r] + hcol[j]];                                  \
-    } else {                                                                   \
-      _Pragma("unroll") for (int j = 0; j < TN; ++j)                           \
-      _Pragma("unroll") for (int r = 0; r < 4; ++r)                            \
-        hv_[j][r] = (float)p.had[hrow_[r] + hcol[j]];                          \
-    }                                                                          \
-    _Pragma("unroll") for (int j = 0; j < TN; ++j)                             \
-    _Pragma("unroll") for (int r = 0; r < 4; ++r) {                            \
-      const long row = m0 + wm * WROWS + i * 16 + 4 * (lane >> 4) + r;         \
+    _Pragma("unroll") for (int j = 0; j < TN; ++j) {                           \
       const long col = n0 + wn * WCOLS + j * 16 + (lane & 15);                 \
-      if (row < M && col < N) {                                                \
-        const long off = roff_[r] + coff[j];                                   \
-        const float val = alpha * hv_[j][r] * acc[i][j][r];                    \
-        STMT;                                                                  \
+      const long row0 = m0 + wm * WROWS + i * 16 + 4 * (lane >> 4);           \
+      float hv_[4];                                                            \
+      if (!had_on) {                                                           \
+        _Pragma("unroll") for (int r = 0; r < 4; ++r) hv_[r] = 1.0f;           \
+      } else if (ht.s) {                                                       \
+        _Pragma("unroll") for (int r = 0; r < 4; ++r) hv_[r] = ht.s[hrow_[r] + hcol[j]]; \
+      } else {                                                                 \
+        _Pragma("unroll") for (int r = 0; r < 4; ++r) hv_[r] = (float)p.had[hrow_[r] + hcol[j]]; \
       }                                                                        \
+      float vv_[4];                                                            \
+      _Pragma("unroll") for (int r = 0; r < 4; ++r) {                          \
+        const long row = row0 + r;                                             \
+        vv_[r] = alpha * hv_[r] * acc[i][j][r];                                \
+        if (row < M && col < N) {                                              \
+          const long off = roff_[r] + coff[j];                                 \
+          const float val = vv_[r];                                            \
+          STMT;                                                                \
+        }                                                                      \
+      }                                                                        \
+      __VA_ARGS__;                                                             \
     }                                                                          \
   }
     if (p.out_type == GEMM_OUT_F64) {
       double* Cd = (double*)p.C;
       if (p.beta != 0.0) {
         const double beta = p.beta;
-        P_EPI(Cd[off] = (double)val + beta * Cd[off]);
+        P_EPI(Cd[off] = (double)val + beta * Cd[off], );
       } else {
-        P_EPI(Cd[off] = (double)val);
+        P_EPI(Cd[off] = (double)val, );
       }
     } else if (p.out_type == GEMM_OUT_F32) {
       float* Cf = (float*)p.C;
       const float beta = (float)p.beta;
-      P_EPI(Cf[off] = beta != 0.0f ? val + beta * Cf[off] : val);
+      P_EPI(Cf[off] = beta != 0.0f ? val + beta * Cf[off] : val, );
     } else {
       u16* Ch = (u16*)p.C;
       u16* Cl = (u16*)p.C2;
-      P_EPI(const u16 h = bf16_bits_rne(val); Ch[off] = h; Cl[off] = bf16_bits_rne(val - bf16_bits_to_float(h)));
+      // optional transposed copy: the lane's four consecutive rows of one column are
+      // consecutive there, one 8-byte store per array (M a multiple of 4)
+      u16* Th = (u16*)p.Ct;
+      u16* Tl = (u16*)p.Ct2;
+      const long tb = b * p.sCt;
+      P_EPI(const u16 h = bf16_bits_rne(val); Ch[off] = h; Cl[off] = bf16_bits_rne(val - bf16_bits_to_float(h)),
+            if (Th && col < N && row0 < M) store_pair4(Th, Tl, tb + col * p.ldct + row0, vv_));
     }
 #undef P_EPI
   }
@@ -611,8 +630,15 @@ int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st) {
   if (p.ksplit > 1 && (p.kchunk & 7)) return DFTB_ERR_ARG;
   if ((((uintptr_t)p.A) | ((uintptr_t)p.A2) | ((uintptr_t)p.B) | ((uintptr_t)p.B2)) & 15) return DFTB_ERR_ARG;
   const bool small = p.tile == 64 || (p.tile == 0 && (p.M < 128 || p.N < 128));
-  const bool pipe = p.tile == 256 || p.tile == 512 ||
-                    (p.tile == 0 && p.M >= 256 && p.N >= 128 && p.K >= 4 * P_BK);
+  bool pipe = p.tile == 256 || p.tile == 512 || (p.tile == 0 && p.M >= 256 && p.N >= 128 && p.K >= 4 * P_BK);
+  if (p.Ct) {
+    // transposed copy: pipelined kernel, 16x16 form, bf16 pairs stored four rows
+    // (8 bytes) per lane, so M, ldct, sCt multiples of 4 and 8-byte aligned arrays
+    if (p.tile == 64 || p.tile == 128 || p.out_type != GEMM_OUT_BF16PAIR || !p.Ct2 || p.ksplit != 1) return DFTB_ERR_ARG;
+    if ((p.M & 3) || (p.ldct & 3) || (p.sCt & 3) || p.ldct < p.M) return DFTB_ERR_ARG;
+    if ((((uintptr_t)p.Ct) | ((uintptr_t)p.Ct2)) & 7) return DFTB_ERR_ARG;
+    pipe = true;
+  }
   if (pipe) {
     // tile 256: 256 x 128 tiles; tile 512: 256 x 256 tiles; automatic: 256 x 256
     // when N >= 256 unless DFTB_BF3_BN=128 asks for the narrower tile
@@ -627,7 +653,8 @@ int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st) {
     // 16x16x32 by default: the kernel is bound by its global->LDS feed, where the
     // 16x16 form measured 2-4% faster (256 x 128 tiles); DFTB_BF3_M32=1 selects the
     // 32x32x16 form
-    static const int m32 = getenv("DFTB_BF3_M32") ? atoi(getenv("DFTB_BF3_M32")) : 0;
+    static const int m32_env = getenv("DFTB_BF3_M32") ? atoi(getenv("DFTB_BF3_M32")) : 0;
+    const int m32 = p.Ct ? 0 : m32_env;
     const dim3 block(DFTB_BF3_WAVES * 64);
     if (bn == 256) {
       if (m32)

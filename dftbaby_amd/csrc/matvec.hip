@@ -28,7 +28,8 @@ struct MvLayout {
 };
 
 // AO exchange intermediates per vector (elements): B1 [2 npad][max(opad, vpad)],
-// BH [2 npad][2 npad], B3 [vpad][2 npad]
+// BH [2 npad][2 npad], B3 [vpad][2 npad]; BF16x3 keeps two BH (BH and its
+// transpose) and two B3 per vector
 static void ao_sizes(const CasidaOperator* op, long* e1, long* eh, long* e3) {
   const long np2 = 2L * op->npad, w = op->opad > op->vpad ? op->opad : op->vpad;
   *e1 = np2 * w;
@@ -77,8 +78,9 @@ static MvLayout mv_layout(const CasidaOperator* op, int nvec, bool hermitian) {
   if (ao) {
     long e1, eh, e3;
     ao_sizes(op, &e1, &eh, &e3);
-    const size_t per_vec = (size_t)(e1 + eh + e3) * ao_esz(op->precision);
-    const size_t budget = (size_t)8 << 30;
+    const int dup = op->precision == 2 ? 2 : 1;
+    const size_t per_vec = (size_t)(e1 + dup * (eh + e3)) * ao_esz(op->precision);
+    const size_t budget = (size_t)16 << 30;
     long c = (long)(budget / (per_vec ? per_vec : 1));
     if (c < 1) c = 1;
     if (c > nvec) c = nvec;
@@ -263,49 +265,67 @@ static int ao_gemm(int prec, const void* A, const void* A2, long lda, const void
 // (G: gamma_lr expanded to orbital pairs, o: Hadamard product), i.e. four GEMMs
 // each over 2 npad-sized AO dimensions instead of the Nat x o x v x (o + v)
 // transition-charge contractions.  Every GEMM is in the K-contiguous (NT) form.
+// G is symmetric, so the bracket of K2 is the transpose of that of K1: in BF16x3
+// mode the K1 bracket BH is written together with its transpose, and K2 starts
+// from that copy (six GEMMs per vector instead of eight).
 static int ao_exchange(const CasidaOperator* op, const MvLayout& L, char* ws, int nvec, const void* V,
-                       const void* Vlo, const void* VT, const void* VTlo, double* X1, double* X2, hipStream_t st) {
+                       const void* Vlo, double* X1, double* X2, hipStream_t st) {
   const int prec = op->precision;
   const long np2 = 2L * op->npad, opad = op->opad, vpad = op->vpad, N = L.N;
   long e1, eh, e3;
   ao_sizes(op, &e1, &eh, &e3);
   const long vs = prec == 0 ? 8 : (prec == 1 ? 4 : 2);   // bytes per vector element (per array)
   const long es = vs;                                     // intermediates use the same element type
+  const int dup = prec == 2 ? 2 : 1;                      // BH, B3 per vector
   int rc;
   for (int n0 = 0; n0 < nvec; n0 += L.aoc) {
     const int c = (nvec - n0 < L.aoc) ? (nvec - n0) : L.aoc;
     char* b1 = (char*)(((uintptr_t)(ws + L.ao) + 255) & ~(uintptr_t)255);
     char* bh = b1 + (size_t)c * e1 * es;
-    char* b3 = bh + (size_t)c * eh * es;
+    char* b3 = bh + (size_t)dup * c * eh * es;
     char *b1l = nullptr, *bhl = nullptr, *b3l = nullptr;
     if (prec == 2) {
-      b1l = b3 + (size_t)c * e3 * es;
+      b1l = b3 + (size_t)dup * c * e3 * es;
       bhl = b1l + (size_t)c * e1 * es;
-      b3l = bhl + (size_t)c * eh * es;
+      b3l = bhl + (size_t)dup * c * eh * es;
     }
     const char* v0 = (const char*)V + (size_t)n0 * N * vs;
     const char* v0l = Vlo ? (const char*)Vlo + (size_t)n0 * N * vs : nullptr;
-    const char* t0 = VT ? (const char*)VT + (size_t)n0 * N * vs : nullptr;
-    const char* t0l = VTlo ? (const char*)VTlo + (size_t)n0 * N * vs : nullptr;
     double* x1 = X1 + (size_t)n0 * N;
     double* x2 = X2 + (size_t)n0 * N;
     // ---- K1: B1 = [Y_v; C_v] V^T, BH = G o ([Y_o; C_o] B1^T), B3 = [C_v; Y_v]^T BH^T, X1 = 1/4 W B3^T
     if ((rc = ao_gemm(prec, op->ao_v[0], op->ao_v[1], vpad, v0, v0l, vpad, N, 1, b1, b1l, opad, e1, (int)np2,
                       (int)opad, (int)vpad, c, 1.0, false, op, false, st))) return rc;
+    if (prec == 2) {
+      // BH and BH^T (= the K2 bracket) for the c vectors, then B3 of both in one launch
+      GemmArgs g = gemm_args_default();
+      g.in_type = GEMM_IN_BF16X3;
+      g.out_type = GEMM_OUT_BF16PAIR;
+      g.A = op->ao_o[0]; g.A2 = op->ao_o[1]; g.lda = opad; g.sA = 0;
+      g.B = b1; g.B2 = b1l; g.ldb = opad; g.sB = e1; g.b_trans = 1;
+      g.C = bh; g.C2 = bhl; g.ldc = np2; g.sC = eh;
+      g.Ct = bh + (size_t)c * eh * es; g.Ct2 = bhl + (size_t)c * eh * es; g.ldct = np2; g.sCt = eh;
+      g.M = (int)np2; g.N = (int)np2; g.K = (int)opad; g.batch = c; g.alpha = 1.0;
+      g.had = op->gam_ao; g.hmod = op->npad; g.ldh = op->nat + 1; g.hmap = op->amap;
+      if ((rc = gemm_launch(g, st))) return rc;
+      if ((rc = ao_gemm(prec, op->ao_cyvt[0], op->ao_cyvt[1], np2, bh, bhl, np2, eh, 1, b3, b3l, np2, e3,
+                        (int)vpad, (int)np2, (int)np2, 2 * c, 1.0, false, op, false, st))) return rc;
+      if ((rc = ao_gemm(prec, op->ao_w[0], op->ao_w[1], np2, b3, b3l, np2, e3, 1, x1, nullptr, vpad, N, (int)opad,
+                        (int)vpad, (int)np2, c, 0.25, true, op, false, st))) return rc;
+      const size_t o3 = (size_t)c * e3 * es;
+      if ((rc = ao_gemm(prec, op->ao_w[0], op->ao_w[1], np2, b3 + o3, b3l + o3, np2, e3, 1, x2, nullptr, vpad, N,
+                        (int)opad, (int)vpad, (int)np2, c, 0.25, true, op, false, st))) return rc;
+      continue;
+    }
     if ((rc = ao_gemm(prec, op->ao_o[0], op->ao_o[1], opad, b1, b1l, opad, e1, 1, bh, bhl, np2, eh, (int)np2,
                       (int)np2, (int)opad, c, 1.0, false, op, true, st))) return rc;
     if ((rc = ao_gemm(prec, op->ao_cyvt[0], op->ao_cyvt[1], np2, bh, bhl, np2, eh, 1, b3, b3l, np2, e3,
                       (int)vpad, (int)np2, (int)np2, c, 1.0, false, op, false, st))) return rc;
     if ((rc = ao_gemm(prec, op->ao_w[0], op->ao_w[1], np2, b3, b3l, np2, e3, 1, x1, nullptr, vpad, N, (int)opad,
                       (int)vpad, (int)np2, c, 0.25, true, op, false, st))) return rc;
-    // ---- K2: B1 = [Y_o; C_o] V, BH = G o ([Y_v; C_v] B1^T), B3, X2 as above
-    if (prec == 2) {
-      if ((rc = ao_gemm(prec, op->ao_o[0], op->ao_o[1], opad, t0, t0l, opad, N, 1, b1, b1l, vpad, e1, (int)np2,
-                        (int)vpad, (int)opad, c, 1.0, false, op, false, st))) return rc;
-    } else {
-      if ((rc = ao_gemm(prec, op->ao_o[0], nullptr, opad, v0, nullptr, vpad, N, 0, b1, nullptr, vpad, e1,
-                        (int)np2, (int)vpad, (int)opad, c, 1.0, false, op, false, st))) return rc;
-    }
+    // ---- K2 (FP64 / FP32): B1 = [Y_o; C_o] V, BH = G o ([Y_v; C_v] B1^T), B3, X2 as above
+    if ((rc = ao_gemm(prec, op->ao_o[0], nullptr, opad, v0, nullptr, vpad, N, 0, b1, nullptr, vpad, e1,
+                      (int)np2, (int)vpad, (int)opad, c, 1.0, false, op, false, st))) return rc;
     if ((rc = ao_gemm(prec, op->ao_v[0], op->ao_v[1], vpad, b1, b1l, vpad, e1, 1, bh, bhl, np2, eh, (int)np2,
                       (int)np2, (int)vpad, c, 1.0, false, op, true, st))) return rc;
     if ((rc = ao_gemm(prec, op->ao_cyvt[0], op->ao_cyvt[1], np2, bh, bhl, np2, eh, 1, b3, b3l, np2, e3,
@@ -365,11 +385,13 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
       DFTB_CHECK_LAUNCH();
       Vx = hi;
       Vlo = lo;
-      // V^T per vector: [n][b][j] (K-contiguous operand of the K2 expand)
-      unsigned short* thi = lo + total;
-      unsigned short* tlo = thi + total;
-      if ((rc = transpose_pair(V, nvec, op->nocc, op->opad, op->vpad, L.N, op->vpad, thi, tlo, L.N, op->opad, st)))
-        return rc;
+      // V^T per vector: [n][b][j] (K-contiguous operand of the transition-charge K2 expand)
+      if (op->lc && op->exchange != 1) {
+        unsigned short* thi = lo + total;
+        unsigned short* tlo = thi + total;
+        if ((rc = transpose_pair(V, nvec, op->nocc, op->opad, op->vpad, L.N, op->vpad, thi, tlo, L.N, op->opad, st)))
+          return rc;
+      }
     }
   }
   if ((rc = coulomb(op, L, ws, Vc, nvec, 4.0, ApBV, st))) return rc;
@@ -391,9 +413,7 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
     const int nat_eff = (op->exchange == 1) ? 0
                         : ((op->atom_limit > 0 && op->atom_limit < op->nat) ? op->atom_limit : op->nat);
     if (op->exchange == 1) {
-      const unsigned short* VTh_ = prec == 2 ? (const unsigned short*)Vlo + total : nullptr;
-      const unsigned short* VTl_ = prec == 2 ? VTh_ + total : nullptr;
-      if ((rc = ao_exchange(op, L, ws, nvec, Vx, Vlo, VTh_, VTl_, X1, X2, st))) return rc;
+      if ((rc = ao_exchange(op, L, ws, nvec, Vx, Vlo, X1, X2, st))) return rc;
     }
     for (int A0 = 0; prec < 2 && A0 < nat_eff; A0 += L.ac) {
       const int ac = (nat_eff - A0 < L.ac) ? (nat_eff - A0) : L.ac;

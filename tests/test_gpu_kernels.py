@@ -182,6 +182,46 @@ def test_gemm_bf16x3_pipelined(M, N, K, batch, tile, rng):
 
 
 @pytest.mark.parametrize("tile", [256, 512])
+def test_gemm_bf16x3_pipelined_transposed_copy(tile, rng):
+    """bf16-pair output written twice: C[b][m][n] and the transposed copy
+    Ct[b * sCt + n * ldct + m] (partial tiles in M and N, strided batches)."""
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _stream
+    M, N, K, batch, ldct = 268, 300, 104, 2, 272
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((batch, N, K))
+    (ah, al), (bh, bl) = _bf_pair(T(A)), _bf_pair(T(B))
+    ch = torch.zeros(batch * M * N, dtype=torch.int16, device=_dev())
+    cl = torch.zeros_like(ch)
+    th = torch.full((batch * N * ldct,), 7, dtype=torch.int16, device=_dev())
+    tl = torch.full_like(th, 7)
+    g = _lib.GemmArgs()
+    g.A, g.A2, g.B, g.B2, g.C, g.C2 = (ah.data_ptr(), al.data_ptr(), bh.data_ptr(), bl.data_ptr(), ch.data_ptr(),
+                                       cl.data_ptr())
+    g.Ct, g.Ct2, g.ldct, g.sCt = th.data_ptr(), tl.data_ptr(), ldct, N * ldct
+    g.lda, g.ldb, g.ldc, g.cs2 = K, K, N, 1
+    g.sA, g.sB, g.sC = 0, N * K, M * N
+    g.M, g.N, g.K, g.batch, g.ksplit = M, N, K, batch, 1
+    g.alpha, g.beta = 1.0, 0.0
+    g.b_trans = 1
+    g.tile = tile
+    g.in_type, g.out_type = _lib.GEMM_IN_BF16X3, _lib.GEMM_OUT_BF16PAIR
+    _lib.call("dftb_gemm", ctypes.byref(g), _stream())
+    f = lambda t: (t.to(torch.int32) & 0xFFFF).cpu().numpy().astype(np.uint32) << 16
+    got = (f(ch).view(np.float32).astype(np.float64) + f(cl).view(np.float32).astype(np.float64)).reshape(batch, M, N)
+    gt = (f(th).view(np.float32).astype(np.float64) + f(tl).view(np.float32).astype(np.float64)).reshape(batch, N, ldct)
+    ref = np.einsum("mk,bnk->bmn", A, B)
+    assert (np.abs(got - ref) <= 3e-5 * np.sqrt(K) + 2.0 ** -15 * np.abs(ref)).all()
+    assert np.array_equal(gt[:, :, :M], got.transpose(0, 2, 1))       # the same pairs, transposed
+    pad = (th.view(batch, N, ldct)[:, :, M:] == 7).all() and (tl.view(batch, N, ldct)[:, :, M:] == 7).all()
+    assert bool(pad)                                                  # columns past M untouched
+    # the copy needs M, ldct, sCt multiples of 4 and the bf16-pair output
+    g.M = M - 2
+    with pytest.raises(_lib.HipError, match="invalid argument"):
+        _lib.call("dftb_gemm", ctypes.byref(g), _stream())
+
+
+@pytest.mark.parametrize("tile", [256, 512])
 @pytest.mark.parametrize("keys", ["atoms", "scattered", "identity"])
 def test_gemm_bf16x3_pipelined_hadamard(tile, keys, rng):
     """Hadamard factor in the epilogue of the pipelined kernel: out[r][c] =
