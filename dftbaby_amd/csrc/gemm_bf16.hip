@@ -221,18 +221,6 @@ __device__ __forceinline__ void dma16(i4_t rs, unsigned lds, unsigned off) {
 // reads have landed, then barrier
 #define P_WAIT_BARRIER(N) asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory")
 
-// Four consecutive values as bf16 (hi, lo) pairs, one 8-byte store per array.
-__device__ __forceinline__ void store_pair4(u16* hi, u16* lo, long off, const float* v) {
-  unsigned h[4], l[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    h[r] = bf16_bits_rne(v[r]);
-    l[r] = bf16_bits_rne(v[r] - bf16_bits_to_float((u16)h[r]));
-  }
-  *(uint2*)(hi + off) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
-  *(uint2*)(lo + off) = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
-}
-
 // Hadamard factor of one output tile, read from LDS.  The factor of element
 // (row, col) is had[ra(row) ldh + cb(col)] with ra, cb the row / column keys
 // (hmap of row % hmod, or row % hmod itself).  After the main loop the ring is
@@ -297,9 +285,10 @@ __device__ HadTile had_stage(const GemmArgs& p, long m0, long n0, u16* ring, int
 // with v_mfma_f32_16x16x32_bf16.  BN = 128: three-stage ring of 48 KiB stages;
 // BN = 256: two-stage ring of 64 KiB stages (the ring must fit the 160 KiB of
 // LDS), 1.5x the MFMA work per byte staged into LDS.
-template <int NW, bool M32, int BN>
+template <int NW, bool M32, int BN, bool TR = false>
 __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
   static_assert(BN == 128 || BN == 256, "pipelined tile width");
+  static_assert(!TR || !M32, "transposed copy: 16x16 form");  // TR: bf16-pair output plus transposed copy
   constexpr int NST = BN == 128 ? 3 : 2;     // ring stages
   constexpr int WMW = NW / 2;                // waves along M
   constexpr int WROWS = P_BM / WMW;          // rows per wave (64 or 128)
@@ -541,7 +530,13 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
   } while (0)
     P_MAINLOOP(P_COMPUTE)
 #undef P_COMPUTE
-    const HadTile ht = had_on ? had_stage<P_BM, BN, NW * 64>(p, m0, n0, ring, NST * STAGE / 2)
+    // transposed copy: each wave stages 16 columns x 64 rows (bf16 hi and lo) at
+    // a time at the end of the ring, after the Hadamard block
+    constexpr int TW_LD = 72;                  // u16 per staged column (64 rows + pad, 16-byte rows)
+    constexpr int TW = 2 * 16 * TW_LD;         // u16 per wave
+    constexpr int TW_OFF = NST * STAGE - NW * TW;
+    const int had_cap = (TR ? TW_OFF : NST * STAGE) / 2;
+    const HadTile ht = had_on ? had_stage<P_BM, BN, NW * 64>(p, m0, n0, ring, had_cap)
                               : HadTile{nullptr, 0, 0, 0};
     // column parts once; row parts per tile row (hoisting all of them would hold
     // 4 TM 64-bit offsets beside the accumulators)
@@ -585,7 +580,8 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
       __VA_ARGS__;                                                             \
     }                                                                          \
   }
-    if (p.out_type == GEMM_OUT_F64) {
+    if (TR) {
+    } else if (p.out_type == GEMM_OUT_F64) {
       double* Cd = (double*)p.C;
       if (p.beta != 0.0) {
         const double beta = p.beta;
@@ -600,13 +596,74 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
     } else {
       u16* Ch = (u16*)p.C;
       u16* Cl = (u16*)p.C2;
-      // optional transposed copy: the lane's four consecutive rows of one column are
-      // consecutive there, one 8-byte store per array (M a multiple of 4)
+      P_EPI(const u16 h = bf16_bits_rne(val); Ch[off] = h; Cl[off] = bf16_bits_rne(val - bf16_bits_to_float(h)), );
+    }
+    if constexpr (TR) {
+      u16* Ch = (u16*)p.C;
+      u16* Cl = (u16*)p.C2;
       u16* Th = (u16*)p.Ct;
       u16* Tl = (u16*)p.Ct2;
       const long tb = b * p.sCt;
-      P_EPI(const u16 h = bf16_bits_rne(val); Ch[off] = h; Cl[off] = bf16_bits_rne(val - bf16_bits_to_float(h)),
-            if (Th && col < N && row0 < M) store_pair4(Th, Tl, tb + col * p.ldct + row0, vv_));
+      {
+        // With the transposed copy the loops run column block outer, so that each
+        // accumulator dies once stored.  The wave's 64 rows of 16 columns go
+        // column-major into its LDS buffer (a lane's four consecutive rows as one
+        // 8-byte write) and are read back as 16-byte row segments of the copy: 8
+        // lanes fill 128 bytes of one transposed row.  The buffer is private to
+        // the wave (no barrier).
+        u16* tw = ring + TW_OFF + wave * TW;
+        const int q = lane >> 4, c = lane & 15;
+        // row parts of the output offset and of the Hadamard index, once per tile
+        long ro[TM][4], hr[TM][4];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const long row = m0 + wm * WROWS + i * 16 + 4 * q + r;
+            ro[i][r] = cbase + gemm_row_part(p, row);
+            hr[i][r] = had_on ? (long)(had_rkey(p, row) - ht.a0) * ht.ld : 0;
+          }
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          const long col = n0 + wn * WCOLS + j * 16 + c;
+#pragma unroll
+          for (int i = 0; i < TM; ++i) {
+            unsigned h[4], l[4];
+            float hv[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              hv[r] = !had_on ? 1.0f : (ht.s ? ht.s[hr[i][r] + hcol[j]] : (float)p.had[hr[i][r] + hcol[j]]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const long row = m0 + wm * WROWS + i * 16 + 4 * q + r;
+              const float val = alpha * hv[r] * acc[i][j][r];
+              h[r] = bf16_bits_rne(val);
+              l[r] = bf16_bits_rne(val - bf16_bits_to_float((u16)h[r]));
+              if (row < M && col < N) {
+                const long off = ro[i][r] + coff[j];
+                Ch[off] = (u16)h[r];
+                Cl[off] = (u16)l[r];
+              }
+            }
+            const int o = c * TW_LD + i * 16 + 4 * q;
+            *(uint2*)(tw + o) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+            *(uint2*)(tw + 16 * TW_LD + o) = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
+          }
+          asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+          for (int t = 0; t < 2; ++t) {
+            const int e = lane + 64 * t, cc = e >> 3, rr = (e & 7) * 8;
+            const uint4 H = *(const uint4*)(tw + cc * TW_LD + rr);
+            const uint4 L = *(const uint4*)(tw + 16 * TW_LD + cc * TW_LD + rr);
+            const long tcol = n0 + wn * WCOLS + j * 16 + cc, trow = m0 + wm * WROWS + rr;
+            if (tcol < N && trow < M) {
+              *(uint4*)(Th + tb + tcol * p.ldct + trow) = H;
+              *(uint4*)(Tl + tb + tcol * p.ldct + trow) = L;
+            }
+          }
+          asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        }
+      }
     }
 #undef P_EPI
   }
@@ -632,11 +689,11 @@ int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st) {
   const bool small = p.tile == 64 || (p.tile == 0 && (p.M < 128 || p.N < 128));
   bool pipe = p.tile == 256 || p.tile == 512 || (p.tile == 0 && p.M >= 256 && p.N >= 128 && p.K >= 4 * P_BK);
   if (p.Ct) {
-    // transposed copy: pipelined kernel, 16x16 form, bf16 pairs stored four rows
-    // (8 bytes) per lane, so M, ldct, sCt multiples of 4 and 8-byte aligned arrays
+    // transposed copy: pipelined kernel, 16x16 form, bf16 pairs stored as 16-byte
+    // row segments (8 rows), so M, ldct, sCt multiples of 8 and 16-byte aligned arrays
     if (p.tile == 64 || p.tile == 128 || p.out_type != GEMM_OUT_BF16PAIR || !p.Ct2 || p.ksplit != 1) return DFTB_ERR_ARG;
-    if ((p.M & 3) || (p.ldct & 3) || (p.sCt & 3) || p.ldct < p.M) return DFTB_ERR_ARG;
-    if ((((uintptr_t)p.Ct) | ((uintptr_t)p.Ct2)) & 7) return DFTB_ERR_ARG;
+    if ((p.M & 7) || (p.ldct & 7) || (p.sCt & 7) || p.ldct < p.M) return DFTB_ERR_ARG;
+    if ((((uintptr_t)p.Ct) | ((uintptr_t)p.Ct2)) & 15) return DFTB_ERR_ARG;
     pipe = true;
   }
   if (pipe) {
@@ -654,9 +711,14 @@ int gemm_bf16x3_launch(const GemmArgs& p, hipStream_t st) {
     // 16x16 form measured 2-4% faster (256 x 128 tiles); DFTB_BF3_M32=1 selects the
     // 32x32x16 form
     static const int m32_env = getenv("DFTB_BF3_M32") ? atoi(getenv("DFTB_BF3_M32")) : 0;
-    const int m32 = p.Ct ? 0 : m32_env;
+    const int m32 = m32_env;
     const dim3 block(DFTB_BF3_WAVES * 64);
-    if (bn == 256) {
+    if (p.Ct) {
+      if (bn == 256)
+        hipLaunchKernelGGL((gemm_bf3_pipe_kernel<DFTB_BF3_WAVES, false, 256, true>), grid, block, 0, st, p);
+      else
+        hipLaunchKernelGGL((gemm_bf3_pipe_kernel<DFTB_BF3_WAVES, false, 128, true>), grid, block, 0, st, p);
+    } else if (bn == 256) {
       if (m32)
         hipLaunchKernelGGL((gemm_bf3_pipe_kernel<DFTB_BF3_WAVES, true, 256>), grid, block, 0, st, p);
       else

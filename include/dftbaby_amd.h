@@ -60,8 +60,8 @@ typedef struct GemmArgs {
   int a_trans, b_trans;
   int in_type, out_type;
   int tile; /* 0 auto, 64 or 128; BF16x3 only: 256 = pipelined 256x128, 512 = pipelined 256x256 */
-  void* Ct; void* Ct2; /* optional transposed copy (pipelined BF16x3, bf16-pair output, no remaps): */
-  long ldct, sCt;      /* element (m, n) of batch b also goes to Ct[b * sCt + n * ldct + m] */
+  void* Ct; void* Ct2; /* optional transposed copy (pipelined BF16x3, bf16-pair output, M, ldct, sCt */
+  long ldct, sCt;      /* multiples of 8): element (m, n) of batch b also goes to Ct[b * sCt + n * ldct + m] */
 } GemmArgs;
 
 int dftb_gemm(const GemmArgs* p, void* stream);

@@ -187,7 +187,7 @@ def test_gemm_bf16x3_pipelined_transposed_copy(tile, rng):
     Ct[b * sCt + n * ldct + m] (partial tiles in M and N, strided batches)."""
     from dftbaby_amd import _lib
     from dftbaby_amd.engine import _stream
-    M, N, K, batch, ldct = 268, 300, 104, 2, 272
+    M, N, K, batch, ldct = 264, 300, 104, 2, 280
     A = rng.standard_normal((M, K))
     B = rng.standard_normal((batch, N, K))
     (ah, al), (bh, bl) = _bf_pair(T(A)), _bf_pair(T(B))
@@ -215,8 +215,8 @@ def test_gemm_bf16x3_pipelined_transposed_copy(tile, rng):
     assert np.array_equal(gt[:, :, :M], got.transpose(0, 2, 1))       # the same pairs, transposed
     pad = (th.view(batch, N, ldct)[:, :, M:] == 7).all() and (tl.view(batch, N, ldct)[:, :, M:] == 7).all()
     assert bool(pad)                                                  # columns past M untouched
-    # the copy needs M, ldct, sCt multiples of 4 and the bf16-pair output
-    g.M = M - 2
+    # the copy needs M, ldct, sCt multiples of 8 and the bf16-pair output
+    g.M = M - 4
     with pytest.raises(_lib.HipError, match="invalid argument"):
         _lib.call("dftb_gemm", ctypes.byref(g), _stream())
 
