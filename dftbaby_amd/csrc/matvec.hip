@@ -265,9 +265,10 @@ static int ao_gemm(int prec, const void* A, const void* A2, long lda, const void
 // (G: gamma_lr expanded to orbital pairs, o: Hadamard product), i.e. four GEMMs
 // each over 2 npad-sized AO dimensions instead of the Nat x o x v x (o + v)
 // transition-charge contractions.  Every GEMM is in the K-contiguous (NT) form.
-// G is symmetric, so the bracket of K2 is the transpose of that of K1: in BF16x3
-// mode the K1 bracket BH is written together with its transpose, and K2 starts
-// from that copy (six GEMMs per vector instead of eight).
+// G is symmetric, so the bracket of K2 is the transpose of that of K1 and K2
+// starts from BH (six GEMMs per vector instead of eight): in BF16x3 mode (NT
+// GEMMs only) the Hadamard GEMM also writes BH^T; FP64 / FP32 read BH in the NN
+// form.
 static int ao_exchange(const CasidaOperator* op, const MvLayout& L, char* ws, int nvec, const void* V,
                        const void* Vlo, double* X1, double* X2, hipStream_t st) {
   const int prec = op->precision;
@@ -323,12 +324,8 @@ static int ao_exchange(const CasidaOperator* op, const MvLayout& L, char* ws, in
                       (int)vpad, (int)np2, (int)np2, c, 1.0, false, op, false, st))) return rc;
     if ((rc = ao_gemm(prec, op->ao_w[0], op->ao_w[1], np2, b3, b3l, np2, e3, 1, x1, nullptr, vpad, N, (int)opad,
                       (int)vpad, (int)np2, c, 0.25, true, op, false, st))) return rc;
-    // ---- K2 (FP64 / FP32): B1 = [Y_o; C_o] V, BH = G o ([Y_v; C_v] B1^T), B3, X2 as above
-    if ((rc = ao_gemm(prec, op->ao_o[0], nullptr, opad, v0, nullptr, vpad, N, 0, b1, nullptr, vpad, e1,
-                      (int)np2, (int)vpad, (int)opad, c, 1.0, false, op, false, st))) return rc;
-    if ((rc = ao_gemm(prec, op->ao_v[0], op->ao_v[1], vpad, b1, b1l, vpad, e1, 1, bh, bhl, np2, eh, (int)np2,
-                      (int)np2, (int)vpad, c, 1.0, false, op, true, st))) return rc;
-    if ((rc = ao_gemm(prec, op->ao_cyvt[0], op->ao_cyvt[1], np2, bh, bhl, np2, eh, 1, b3, b3l, np2, e3,
+    // ---- K2 (FP64 / FP32): B3 = [C_v; Y_v]^T BH (NN form: the K2 bracket is BH^T), X2 as above
+    if ((rc = ao_gemm(prec, op->ao_cyvt[0], op->ao_cyvt[1], np2, bh, bhl, np2, eh, 0, b3, b3l, np2, e3,
                       (int)vpad, (int)np2, (int)np2, c, 1.0, false, op, false, st))) return rc;
     if ((rc = ao_gemm(prec, op->ao_w[0], op->ao_w[1], np2, b3, b3l, np2, e3, 1, x2, nullptr, vpad, N, (int)opad,
                       (int)vpad, (int)np2, c, 0.25, true, op, false, st))) return rc;
