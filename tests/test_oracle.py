@@ -178,6 +178,12 @@ def test_ao_exchange_identity(name, lc, rng):
     M = omega * V - X1 + X2
     assert np.abs(P - refP).max() < 1e-13 * np.abs(refP).max()
     assert np.abs(M - refM).max() < 1e-13 * np.abs(refM).max()
+    # the six-GEMM form the GPU runs: the K2 bracket is the transpose of the K1
+    # bracket (gamma_lr symmetric), so X2 follows from BH in the NN form
+    BH = G2 * (Ao @ (Av @ V.T).T)
+    assert np.abs(G2 * (Av @ (Ao @ V).T) - BH.T).max() < 1e-13 * np.abs(BH).max()
+    X2s = 0.25 * W @ (CYvT @ BH).T
+    assert np.abs(X2s - X2).max() < 1e-13 * np.abs(X2).max()
 
 
 def test_casida_self_consistency():
