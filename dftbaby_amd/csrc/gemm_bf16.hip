@@ -180,16 +180,16 @@ __global__ __launch_bounds__(256) void gemm_bf3_kernel(GemmArgs p) {
 }
 
 // ---------------------------------------------------------------------------
-// Large-tile pipelined variant: 256 x 128 output tile per workgroup (four
-// wave64s, each 128 x 64 = 8 x 4 MFMA tiles), BK = 32, three-stage LDS ring
-// filled by direct buffer->LDS loads (buffer_load_dwordx4 ... lds).  Each wave
-// instruction deposits 64 x 16 bytes contiguously in LDS; the lanes choose
-// their global 16-byte chunk so that LDS row r holds k-chunk c in slot
-// c ^ ((r >> 2) & 3), which makes the 16-byte fragment reads of 16 rows at one
-// k-chunk hit disjoint banks.  Out-of-range rows and k-chunks get an offset
-// past the buffer's num_records and load zeros.  The ring stages are separate
-// LDS arrays so the compiler's own waits for the DMA stay precise; the
-// cross-wave hand-off is an explicit vmcnt wait followed by s_barrier.
+// Large-tile pipelined variant: 256 x 256 (or 256 x 128) output tile per
+// workgroup of eight wave64s (4 along M x 2 along N), BK = 32, a two-stage (or
+// three-stage) LDS ring filled by direct buffer->LDS loads (buffer_load_dwordx4
+// ... lds).  Each wave instruction deposits 64 x 16 bytes contiguously in LDS;
+// the lanes choose their global 16-byte chunk so that LDS row r holds k-chunk c
+// in slot c ^ ((r >> 2) & 3), which makes the 16-byte fragment reads of 16 rows
+// at one k-chunk hit disjoint banks.  Out-of-range rows and k-chunks get an
+// offset past the buffer's num_records and load zeros.  The DMA is issued as
+// inline asm so the compiler does not wait for it; the cross-wave hand-off is
+// an explicit vmcnt wait followed by s_barrier.
 #define P_BM 256
 #define P_BK 32
 
