@@ -532,7 +532,7 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
 #undef P_COMPUTE
     // transposed copy: each wave stages 16 columns x 64 rows (bf16 hi and lo) at
     // a time at the end of the ring, after the Hadamard block
-    constexpr int TW_LD = 72;                  // u16 per staged column (64 rows + pad, 16-byte rows)
+    constexpr int TW_LD = 72;                  // u16 per staged column (WROWS <= 64 rows + pad, 16-byte rows)
     constexpr int TW = 2 * 16 * TW_LD;         // u16 per wave
     constexpr int TW_OFF = NST * STAGE - NW * TW;
     const int had_cap = (TR ? TW_OFF : NST * STAGE) / 2;
@@ -606,7 +606,7 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
       const long tb = b * p.sCt;
       {
         // With the transposed copy the loops run column block outer, so that each
-        // accumulator dies once stored.  The wave's 64 rows of 16 columns go
+        // accumulator dies once stored.  The wave's WROWS rows of 16 columns go
         // column-major into its LDS buffer (a lane's four consecutive rows as one
         // 8-byte write) and are read back as 16-byte row segments of the copy: 8
         // lanes fill 128 bytes of one transposed row.  The buffer is private to
@@ -650,9 +650,11 @@ __global__ __launch_bounds__(NW * 64) void gemm_bf3_pipe_kernel(GemmArgs p) {
             *(uint2*)(tw + 16 * TW_LD + o) = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
           }
           asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+          // 16 columns x WROWS / 8 segments of 8 rows, 64 per pass
+          constexpr int SEG = WROWS / 8;
 #pragma unroll
-          for (int t = 0; t < 2; ++t) {
-            const int e = lane + 64 * t, cc = e >> 3, rr = (e & 7) * 8;
+          for (int t = 0; t < WROWS / 32; ++t) {
+            const int e = lane + 64 * t, cc = e / SEG, rr = (e % SEG) * 8;
             const uint4 H = *(const uint4*)(tw + cc * TW_LD + rr);
             const uint4 L = *(const uint4*)(tw + 16 * TW_LD + cc * TW_LD + rr);
             const long tcol = n0 + wn * WCOLS + j * 16 + cc, trow = m0 + wm * WROWS + rr;
