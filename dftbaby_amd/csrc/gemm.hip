@@ -276,6 +276,7 @@ int gemm_launch(const GemmArgs& p0, hipStream_t st) {
   if (p.ksplit <= 0) p.ksplit = 1;
   if (p.kchunk <= 0) p.kchunk = p.K;
   if (p.out_type == GEMM_OUT_BF16PAIR && !p.C2) return DFTB_ERR_ARG;
+  if (p.had && (p.hmod <= 0 || p.ldh <= 0)) return DFTB_ERR_ARG;  // Hadamard keys are taken modulo hmod
   const bool big = (p.M >= 128 && p.N >= 128) || p.tile == 128;
   const bool small = p.tile == 64 || !big;
   if (p.in_type == GEMM_IN_BF16X3) return gemm_bf16x3_launch(p, st);

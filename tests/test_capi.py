@@ -42,3 +42,32 @@ def test_product_path_does_not_import_oracle():
                 src = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in src and "from oracle" not in src, f
                 assert "tddftb_oracle" not in src, f
+
+
+def test_gemm_rejects_invalid_transposed_copy_and_hadamard_arguments():
+    """Argument checks of dftb_gemm that return before any device work: a
+    Hadamard factor needs hmod > 0; the transposed copy needs the BF16x3 input,
+    the bf16-pair output and M a multiple of 8."""
+    from dftbaby_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("HIP library not built")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    fake = 1 << 20                                   # never dereferenced: every case fails its checks first
+
+    def args(**kw):
+        g = _lib.GemmArgs()
+        g.A = g.A2 = g.B = g.B2 = g.C = g.C2 = fake
+        g.lda = g.ldb = g.K = 64
+        g.ldc, g.M, g.N, g.batch, g.ksplit, g.kchunk = 64, 64, 64, 1, 1, 64
+        g.b_trans, g.alpha, g.hmod = 1, 1.0, 1
+        g.in_type, g.out_type = _lib.GEMM_IN_BF16X3, _lib.GEMM_OUT_BF16PAIR
+        for k, v in kw.items():
+            setattr(g, k, v)
+        return g
+
+    cases = [args(had=fake, hmod=0, ldh=8),                                    # Hadamard modulus 0
+             args(Ct=fake, Ct2=fake, ldct=64, sCt=4096, out_type=_lib.GEMM_OUT_F64),   # copy needs pair output
+             args(Ct=fake, Ct2=fake, ldct=64, sCt=4096, M=60),                 # M not a multiple of 8
+             args(Ct=fake, Ct2=fake, ldct=64, sCt=4096, in_type=_lib.GEMM_IN_F64, b_trans=1)]  # FP64 kernel
+    for g in cases:
+        assert lib.dftb_gemm(ctypes.byref(g), None) == 1
