@@ -10,10 +10,11 @@ producer in ``dftbaby_amd.groundstate``.
 
 Every excited-state pass runs on the GPU: transition charges (kernel (a)), the
 Casida matvecs (b) and the Davidson loop (c), Oia / Lambda2 and the transition
-dipoles.  Out of scope (raise NotImplementedError): multipole charges
-(mulliken_dipoles=1), symmetry selection, the CT correction, the stability
-analysis and Tamm-Dancoff (which the reference itself disables,
-Solver.py:194).
+dipoles.  The long-range CT correction (ct_correction=1, lc = 0 only) shifts
+the diagonal as in LR_TDDFTB.py:429-437 (``dftbaby_amd.LongRangeCT``).  Out of
+scope (raise NotImplementedError): multipole charges (mulliken_dipoles=1),
+symmetry selection, the stability analysis and Tamm-Dancoff (which the
+reference itself disables, Solver.py:194).
 """
 import time
 
@@ -228,8 +229,6 @@ class LR_TDDFTB:
         """LR_TDDFTB.py:403-527 on the GPU."""
         if stability == 1:
             raise NotImplementedError("the ground-state stability analysis is out of scope")
-        if ct_correction == 1:
-            raise NotImplementedError("the long-range CT correction (ct_correction=1) is out of scope")
         if response_method == "Tamm-Dancoff":
             raise Exception("Revise the TDA code!!!!")   # Solver.py:194: disabled in the reference
         if response_method != "Casida":
@@ -238,13 +237,23 @@ class LR_TDDFTB:
         op = self.op
         lc = op.lc
         prec = precision_code(precision)
+        # long-range CT correction: shifts of the orbital-energy differences (LR_TDDFTB.py:429-437)
+        self.ct_correction = ct_correction
+        omega_shift = None
+        if ct_correction == 1:
+            assert lc == 0, ("CT correction --ct_correction=1 cannot be combined with long-range correction "
+                             "--long_range_correction=1!")
+            from .LongRangeCT import LongRangeCTCorrection
+            omega_shift = LongRangeCTCorrection(self).diagonal_shifts(self.omega,
+                                                                       verbose=getattr(self.dftb2, "verbose", 0))
+        self.omega_shift = omega_shift
         verbose = getattr(self.dftb2, "verbose", 0)
         t0 = time.time()
         if nstates is None:
-            w, C, XmY, XpY = engine.casida_dense_gpu(op)
+            w, C, XmY, XpY = engine.casida_dense_gpu(op, omega_shift=omega_shift)
         elif lc == 0:
             w, C, XmY, XpY, st = engine.hermitian_davidson_gpu(
-                op, nstates, omega_shift=None, ifact=diag_ifact, conv=diag_conv, maxiter=diag_maxiter,
+                op, nstates, omega_shift=omega_shift, ifact=diag_ifact, conv=diag_conv, maxiter=diag_maxiter,
                 XpYguess=self.XpY, Oia=self.Oia, L2_thresh=diag_L2threshold, precision=prec, verbose=verbose)
             self.davidson_stats = st
         else:
@@ -272,7 +281,7 @@ class LR_TDDFTB:
         self._Cij_h = self._XmY_h = self._XpY_h = None
         self.Omega = w.cpu().numpy()
         if diag_check == 1 and nstates is not None:
-            Om_full, C_full, XmY_full, XpY_full = engine.casida_dense_gpu(op)
+            Om_full, C_full, XmY_full, XpY_full = engine.casida_dense_gpu(op, omega_shift=omega_shift)
             Om_full = Om_full.cpu().numpy()
             err = np.sum(np.abs(self.Omega - Om_full[:nstates]))
             assert err < 1.0e-10, "eigenvalues differ, err = %s" % err

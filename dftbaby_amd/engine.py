@@ -940,8 +940,10 @@ def hermitian_davidson_gpu(op, nstates, omega_shift=None, ifact=1, conv=1.0e-14,
     return w, C, XmY, XpY, stats
 
 
-def casida_dense_gpu(op, chunk=512):
+def casida_dense_gpu(op, chunk=512, omega_shift=None):
     """Full Casida solve (Solver.py:98-181) for small systems: all n_ov roots.
+    omega_shift (o x v, the CT correction) is added to the diagonals of A and B
+    (Solver.py:46-49), i.e. twice to A+B.
 
     (A+-B) are assembled column block by column block with the matvec kernel,
     (A-B)^{1/2} is diagonal (lc = 0) or the Newton-Schulz square root, and the
@@ -964,6 +966,9 @@ def casida_dense_gpu(op, chunk=512):
         P, M = op.matvec(V, 0)
         ApB[c0:c0 + m] = P[:, real]
         AmB[c0:c0 + m] = M[:, real]
+    if omega_shift is not None:
+        sh = torch.as_tensor(np.asarray(omega_shift, dtype=np.float64).reshape(-1), device=dev)
+        ApB.diagonal().add_(2.0 * sh)
     ws = op.ws
     dAmB = torch.diagonal(AmB).contiguous()
     offdiag = float(torch.linalg.norm(AmB - torch.diag(dAmB)))

@@ -26,6 +26,10 @@ Pinning (see tests/test_oracle.py):
   oscillator strengths - closed form f = 2/3 Omega |d|^2 checked against an
                          independent dense evaluation; Mulliken variant from q.
   Oia / Lambda2        - brute-force double loop of LR_TDDFTB.py:538-560.
+  ct_shifts            - separated fragments: S_ia -> 0 and R_ia -> the distance
+                         between the centres of two centrosymmetric fragments,
+                         so T_ia reduces to the closed form of the asymptotic
+                         correction; local excitations (large S_ia) get T = 0.
 """
 import numpy as np
 import scipy.linalg as sla
@@ -413,6 +417,37 @@ def Lambda2_calcOia_loop(q_oo, q_vv, gaussOmega):
             Oaa = q_vv[:, a, a] @ (gaussOmega @ q_vv[:, a, a])
             Oia[i, a] = q_oo[:, i, i] @ (gaussOmega @ q_vv[:, a, a]) / np.sqrt(Oii * Oaa)
     return Oia
+
+
+def ct_shifts(q_oo, q_vv, gaussOmega, positions, orbe_virt, omega, Sc=0.0001, Rc=0.1):
+    """Long-range CT correction T_ia of LongRangeCT.py:13-76 (Neugebauer et al.,
+    J. Chem. Phys. 124, 214102 (2006)): S_ia = q_ii^T Omega q_aa (19-35), the
+    distance R_ia of the Mulliken orbital centres (37-49), D_ia = -eps_a (53-57),
+    T_ia = exp(-(S/Sc)^2) exp(-(Rc/R)^2) (D - 1/R + (D - 1/R)^2 / (2 w)) clipped
+    to [0, 1e10] (59-68)."""
+    nocc, nvirt = omega.shape
+    Nat = q_oo.shape[0]
+    qi = np.zeros((Nat, nocc))
+    qa = np.zeros((Nat, nvirt))
+    for i in range(nocc):
+        qi[:, i] = q_oo[:, i, i]
+    for a in range(nvirt):
+        qa[:, a] = q_vv[:, a, a]
+    Sia = np.dot(qi.T, np.dot(gaussOmega, qa))
+    Xia = np.zeros((nocc, nvirt))
+    Yia = np.zeros((nocc, nvirt))
+    Zia = np.zeros((nocc, nvirt))
+    for A in range(Nat):
+        xA, yA, zA = positions[A]
+        dqia = np.outer(qi[A, :], np.ones(nvirt)) - np.outer(np.ones(nocc), qa[A, :])
+        Xia += dqia * xA
+        Yia += dqia * yA
+        Zia += dqia * zA
+    Ria = np.sqrt(Xia ** 2 + Yia ** 2 + Zia ** 2)
+    Dia = -np.outer(np.ones(nocc), orbe_virt)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        Tia = np.exp(-(Sia / Sc) ** 2) * np.exp(-(Rc / Ria) ** 2) * (Dia - 1.0 / Ria + (Dia - 1.0 / Ria) ** 2 / (2 * omega))
+    return np.clip(Tia, 0.0, 1.0e10), Sia, Ria
 
 
 def Lambda2(Cij, Oia):

@@ -109,3 +109,45 @@ def test_error_paths():
     with pytest.raises(ExcitedStatesError):
         Solver.nonHermitianDavidson(gs.getGamma(), glr, q_oo, q_vv, q_ov, omega, len(ao), len(av), None, None,
                                     None, nstates=4, conv=1e-8, maxiter=50, verbose=0)
+
+
+@pytest.mark.parametrize("ifact", [None, 1, 4])
+def test_ct_correction(ifact):
+    """ct_correction=1 (LR_TDDFTB.py:429-437) on C8H10 + H2 30 bohr apart: the
+    shifts of LongRangeCT.py:13-93 computed on the device equal the oracle's; the
+    dense path (ifact None) equals the oracle's dense Casida with the same shifts;
+    the Davidson (14 roots) equals the oracle's HermitianDavidson with the shifts
+    (ifact 1: the block-diagonal problem of the separated fragments leaves two
+    roots out of reach of the 14 starting vectors, as in the reference) and, from
+    56 starting vectors, the dense spectrum, which the shift moves by 0.013 Ha."""
+    from dftbaby_amd.LR_TDDFTB import LR_TDDFTB
+    gs = ground_state("C8H10-H2", 0)
+    gs.verbose = 0
+    f, orbe, C, S = gs.getOccupation(), gs.getKSEnergies(), gs.getKSCoefficients(), gs.getOverlapMatrix()
+    _, _, ao, av = O.active_orbitals(f, orbe)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    omega, df = O.orbital_differences(orbe, f, ao, av)
+    pos = np.array([p for _, p in gs.getGeometry()])
+    T, _, _ = O.ct_shifts(q_oo, q_vv, gs.getGaussianOverlap(), pos, orbe[av], omega)
+    g = gs.getGamma()
+    o, v = omega.shape
+    td = LR_TDDFTB(ground_state=gs)
+    if ifact is None:
+        Omega, _ = td.getEnergies(nstates=None, ct_correction=1)
+        ref = O.Casida(g, 0 * g, q_oo, q_vv, q_ov, omega, T, df, o, v)[0]
+    else:
+        k = 14
+        Omega, _ = td.getEnergies(nstates=k, ct_correction=1, diag_ifact=ifact, diag_conv=1e-11, diag_maxiter=200)
+        Oia = O.Lambda2_calcOia(q_oo, q_vv, gs.getGaussianOverlap())
+        ref = O.HermitianDavidson(g, q_ov, omega, T, o, v, None, None, Oia, nstates=k, ifact=ifact, conv=1e-11,
+                                  maxiter=200, L2_thresh=0.0)[0]
+        if ifact == 4:
+            dense = O.Casida(g, 0 * g, q_oo, q_vv, q_ov, omega, T, df, o, v)[0]
+            dense0 = O.Casida(g, 0 * g, q_oo, q_vv, q_ov, omega, 0 * T, df, o, v)[0]
+            np.testing.assert_allclose(Omega, dense[:k], atol=1e-10)
+            assert np.abs(dense[:k] - dense0[:k]).max() > 0.01
+    np.testing.assert_allclose(td.omega_shift, T, atol=1e-12)
+    assert T.max() > 0.04
+    np.testing.assert_allclose(Omega, ref[:len(Omega)], atol=1e-10)
+    with pytest.raises(AssertionError):
+        LR_TDDFTB(ground_state=ground_state("C8H10", 1)).getEnergies(nstates=2, ct_correction=1)

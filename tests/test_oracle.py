@@ -268,3 +268,34 @@ def test_mlp_oracle_gradient_finite_differences():
         e[i] = h
         fd = (mlp_oracle.loss_and_grad(p + e, shapes, X, T, w)[0] - mlp_oracle.loss_and_grad(p - e, shapes, X, T, w)[0]) / (2 * h)
         assert abs(fd - g[i]) < 1e-6 * max(1.0, abs(fd))
+
+
+def test_ct_shifts_separated_fragments():
+    """Long-range CT correction (LongRangeCT.py:13-76) on C8H10 + H2 30 bohr apart:
+    every orbital lives on one centrosymmetric fragment, so an excitation between
+    the fragments has S_ia -> 0 and R_ia = the distance of the fragment centres
+    (30 bohr), and T_ia must equal the asymptotic form exp(-(Rc/R)^2) (D - 1/R +
+    (D - 1/R)^2 / (2 w)) with D = -eps_a (clipped at 0); excitations inside a
+    fragment have R_ia ~ 0 (1e-5 bohr: the SCC density is symmetric to that
+    level) and get no shift."""
+    gs = ground_state("C8H10-H2", 0)
+    f, orbe, C, S, ao, av = _active(gs)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    omega, _ = O.orbital_differences(orbe, f, ao, av)
+    pos = np.array([p for _, p in gs.getGeometry()])
+    T, Sia, Ria = O.ct_shifts(q_oo, q_vv, gs.getGaussianOverlap(), pos, orbe[av], omega)
+    # fragment of each orbital from its Mulliken weight on the two H2 atoms (the last two)
+    w_h2 = np.einsum("mp,mp->p", C[-2:], (S @ C)[-2:])
+    assert np.all((w_h2 < 1e-8) | (w_h2 > 1 - 1e-8))
+    on_h2 = w_h2 > 0.5
+    cross = on_h2[ao][:, None] != on_h2[av][None, :]
+    R = np.linalg.norm(pos[-2:].mean(axis=0) - pos[:-2].mean(axis=0))
+    assert abs(R - 30.0) < 1e-12
+    np.testing.assert_allclose(Ria[cross], R, rtol=1e-6)
+    assert np.abs(Sia[cross]).max() < 1e-30
+    D = -np.broadcast_to(orbe[av][None, :], omega.shape)[cross]
+    x = D - 1.0 / R
+    closed = np.clip(np.exp(-(0.1 / R) ** 2) * (x + x ** 2 / (2 * omega[cross])), 0.0, None)
+    np.testing.assert_allclose(T[cross], closed, rtol=1e-6, atol=1e-14)
+    assert T[cross].max() > 0.04                       # H2 sigma -> C8H10 LUMO is shifted
+    assert np.all(T[~cross] == 0.0) and np.all(Ria[~cross] < 1e-4)
