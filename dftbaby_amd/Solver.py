@@ -8,12 +8,13 @@ out), computed on the GPU by ``dftbaby_amd.engine``.  The extra keyword
 eigensolve are FP64 in every mode.
 
 Reference: HermitianDavidson Solver.py:404-578, nonHermitianDavidson
-Solver.py:637-917, exceptions Solver.py:27-33.
+Solver.py:637-917, Krylov_solver_Zvector Solver.py:921-1012, exceptions
+Solver.py:27-33.
 """
 import numpy as np
 
 from .engine import (CasidaOperatorGPU, ExcitedStatesError, ExcitedStatesNotConverged,  # noqa: F401
-                     hermitian_davidson_gpu, nonhermitian_davidson_gpu)
+                     hermitian_davidson_gpu, nonhermitian_davidson_gpu, zvector_gpu)
 
 _PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, 0: 0, 1: 1, 2: 2}
 
@@ -58,3 +59,27 @@ def nonHermitianDavidson(g0, g0_lr, q_oo, q_vv, q_ov, omega, nocc, nvirt, XmYgue
                                                   precision=precision_code(precision), verbose=verbose)
     C, XmY, XpY = _out(op, (C, XmY, XpY))
     return w.cpu().numpy(), C, XmY, XpY
+
+
+def Krylov_solver_Zvector(A, Adiag, B, X0, maxiter=1000, conv=1.0e-14, verbose=1, precision="fp64"):
+    """Solve (A+B) X = B iteratively (Solver.py:921-1012), as ExcGradients.py:627-635
+    calls it for the Z-vector equations.
+
+    ``A`` is the (A+B) operator: the reference passes a closure over
+    ``_ApBv(g0, g0_lr, qoo, qov, qvv, omega, vs, 1)``; here it is the
+    ``CasidaOperatorGPU`` built from the same inputs (``CasidaOperatorGPU.from_charges``
+    or the operator of an ``LR_TDDFTB`` object), whose products run on the GPU.
+    ``Adiag`` is the orbital-energy difference omega (nocc, nvirt) of that operator
+    (the preconditioner).  B: (nocc, nvirt, k); X0: (nocc, nvirt, k) or None.
+    Returns X (nocc, nvirt, k); raises Exception when it does not converge.
+    """
+    if not isinstance(A, CasidaOperatorGPU):
+        raise TypeError("Krylov_solver_Zvector: A must be the CasidaOperatorGPU of the (A+B) products")
+    Ad = np.asarray(Adiag, dtype=np.float64)
+    if not np.allclose(Ad, A.omega_real[:A.nocc, :A.nvirt].cpu().numpy(), rtol=0.0, atol=1e-12):
+        raise ValueError("Krylov_solver_Zvector: Adiag must be the operator's orbital-energy differences")
+    Bk = np.moveaxis(np.asarray(B, dtype=np.float64), 2, 0)
+    X0k = None if X0 is None else np.moveaxis(np.asarray(X0, dtype=np.float64), 2, 0)
+    X = zvector_gpu(A, Bk, X0=X0k, maxiter=maxiter, conv=conv, precision=precision_code(precision),
+                    verbose=verbose)
+    return np.moveaxis(X, 0, 2)

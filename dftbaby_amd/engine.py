@@ -1015,3 +1015,102 @@ def casida_dense_gpu(op, chunk=512, omega_shift=None):
         out[:, real] = Xc.T
         return out
     return Om, padded(Cc), padded(XmYc), padded(XpYc)
+
+
+def zvector_gpu(op, B, X0=None, maxiter=1000, conv=1.0e-14, precision=0, verbose=0):
+    """Solve (A+B) X = B for k right-hand sides on the GPU (Solver.py:921-1012,
+    the Z-vector equations of ExcGradients.py:627-635).
+
+    Block Krylov (Galerkin) iteration: the basis starts from diag(omega)^{-1} B (or
+    X0), the projected system bs^T (A+B) bs Xb = bs^T B is solved through the GPU
+    eigensolver (it is symmetric positive definite), X = bs^T Xb, the residual
+    (A+B) X - B comes from the stored products (no extra matvec), and each
+    residual above 0.01 conv, preconditioned by diag(omega)^{-1}, expands the basis.
+    The basis is kept orthonormal (same span, hence the same X, as the reference's
+    Gram-Schmidt of the whole set).  B and X0 are (k, o, v); returns X (k, o, v).
+    Raises Exception (the reference's) when maxiter iterations do not converge.
+    """
+    dev = op.device
+    st = _stream()
+    ws = op.ws
+    N, o, v = op.N, op.nocc, op.nvirt
+    Bd = op.to_padded(B)
+    k = Bd.shape[0]
+    kmax = o * v
+    ainv = torch.where(op.omega_real.reshape(-1) != 0, 1.0 / op.omega, torch.zeros_like(op.omega))
+    cap = min(kmax, 64 * k)
+    bs = torch.zeros((cap, N), dtype=torch.float64, device=dev)
+    P = torch.zeros((cap, N), dtype=torch.float64, device=dev)
+
+    def grow(need):
+        nonlocal bs, P, cap
+        if need <= cap:
+            return
+        new = min(kmax, max(need, 2 * cap))
+        for name in ("bs", "P"):
+            old = bs if name == "bs" else P
+            t = torch.zeros((new, N), dtype=torch.float64, device=dev)
+            t[:cap].copy_(old)
+            if name == "bs":
+                bs = t
+            else:
+                P = t
+        cap = new
+
+    if X0 is None:
+        call("dftb_scale2", _p(Bd), None, _p(ainv), k, N, _p(bs[:k]), st)
+    else:
+        bs[:k].copy_(op.to_padded(X0))
+    l = _orthonormalize_block(bs, 0, k, N, ws, st)
+    op.matvec(bs[:l], precision, out_p=P[:l])
+    norms = torch.empty(k, dtype=torch.float64, device=dev)
+    X = torch.empty((k, N), dtype=torch.float64, device=dev)
+    W = torch.empty((k, N), dtype=torch.float64, device=dev)
+    t0 = time.time()
+    for it in range(maxiter):
+        Ab = torch.empty((l, l), dtype=torch.float64, device=dev)
+        gram(bs, l, P, l, N, Ab, ws)
+        Bb = torch.empty((l, k), dtype=torch.float64, device=dev)
+        gram(bs, l, Bd, k, N, Bb, ws)
+        asym = torch.zeros(1, dtype=torch.float64, device=dev)
+        call("dftb_symmetrize", _p(Ab), l, l, _p(asym), st)
+        # Xb = V diag(1/lambda) V^T Bb with Ab = V diag(lambda) V^T (GPU eigensolver)
+        lam = torch.empty(l, dtype=torch.float64, device=dev)
+        V = torch.empty((l, l), dtype=torch.float64, device=dev)
+        w_, wn_ = ws.get(_lib.load().dftb_syevx_workspace(l, l))
+        call("dftb_syevx", _p(Ab), l, l, l, _p(lam), _p(V), _p(w_), wn_, st)
+        if float(lam.min().cpu()) <= 0.0:
+            raise ExcitedStatesError("Z-vector equations: the projected (A+B) is not positive definite")
+        C1 = torch.empty((l, k), dtype=torch.float64, device=dev)
+        gemm(V, Bb, C1, l, k, l, l, k, k, a_trans=1)
+        call("dftb_scale2", _p(C1), _p(1.0 / lam), None, l, k, _p(C1), st)
+        Xb = torch.empty((l, k), dtype=torch.float64, device=dev)
+        gemm(V, C1, Xb, l, k, l, l, k, k)
+        gemm(Xb, bs, X, k, N, l, k, N, N, a_trans=1)
+        gemm(Xb, P, W, k, N, l, k, N, N, a_trans=1)              # (A+B) X from the stored products
+        W.sub_(Bd)
+        call("dftb_row_norms", _p(W), k, N, _p(norms), st)
+        nr = norms.cpu().numpy()
+        if verbose > 0:
+            print("Iteration %d: ( %d expansion vectors )   time: %2.6f s" % (it, l, time.time() - t0))
+            for n in range(k):
+                print("  Solution vector  %3d      %s   res. norm = %e"
+                      % (n, "converged    " if nr[n] < conv else "not converged", nr[n]))
+        if (nr < conv).all():
+            if verbose > 0:
+                print("Z-vector equations CONVERGED (residual norms < %s) after %d iterations" % (conv, it + 1))
+            return op.from_padded(X).cpu().numpy().copy()
+        dkmax = min(kmax - l, k)
+        sel = [n for n in range(dkmax) if nr[n] > 0.01 * conv]
+        if not sel:
+            break
+        grow(l + len(sel))
+        idx = torch.as_tensor(sel, dtype=torch.int64, device=dev)
+        call("dftb_scale2", _p(W[idx].contiguous()), None, _p(ainv), len(sel), N, _p(bs[l:l + len(sel)]), st)
+        l_new = _orthonormalize_block(bs, l, len(sel), N, ws, st)
+        if l_new == l:
+            break
+        op.matvec(bs[l:l_new], precision, out_p=P[l:l_new])
+        l = l_new
+    raise Exception("Z-vector equations: Krylov solver for linear system did not converge after %s iterations "
+                    "(threshold: %e) !!!" % (it + 1, conv))

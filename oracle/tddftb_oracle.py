@@ -26,6 +26,7 @@ Pinning (see tests/test_oracle.py):
   oscillator strengths - closed form f = 2/3 Omega |d|^2 checked against an
                          independent dense evaluation; Mulliken variant from q.
   Oia / Lambda2        - brute-force double loop of LR_TDDFTB.py:538-560.
+  Krylov_solver_Zvector- the dense solution (A+B)^{-1} B of buildAB's matrices.
   ct_shifts            - separated fragments: S_ia -> 0 and R_ia -> the distance
                          between the centres of two centrosymmetric fragments,
                          so T_ia reduces to the closed form of the asymptotic
@@ -417,6 +418,57 @@ def Lambda2_calcOia_loop(q_oo, q_vv, gaussOmega):
             Oaa = q_vv[:, a, a] @ (gaussOmega @ q_vv[:, a, a])
             Oia[i, a] = q_oo[:, i, i] @ (gaussOmega @ q_vv[:, a, a]) / np.sqrt(Oii * Oaa)
     return Oia
+
+
+def gram_schmidt_2d(W):
+    """Modified Gram-Schmidt of the columns of W (nocc, nvirt, n), Solver.py:291-340
+    (the linear-dependence pruning of 317-323 is not needed by the callers here)."""
+    V = np.zeros(W.shape)
+    V[:, :, 0] = W[:, :, 0] / _norm(W[:, :, 0])
+    for n in range(1, W.shape[-1]):
+        V[:, :, n] = np.copy(W[:, :, n])
+        for m in range(n):
+            V[:, :, n] -= np.sum(V[:, :, m] * V[:, :, n]) * V[:, :, m]
+        V[:, :, n] /= _norm(V[:, :, n])
+    return V
+
+
+def Krylov_solver_Zvector(A, Adiag, B, X0, maxiter=1000, conv=1.0e-14):
+    """A.X = B by the Krylov (Galerkin) iteration of Solver.py:921-1012: basis
+    Ainv B (or X0), projected solve, residual, Ainv-preconditioned residuals as new
+    vectors, Gram-Schmidt of the whole set."""
+    nocc, nvirt, k = B.shape
+    kmax = nocc * nvirt
+    l = k
+    Ainv = 1.0 / Adiag
+    if X0 is None:
+        bs = np.zeros((nocc, nvirt, k))
+        for n in range(k):
+            bs[:, :, n] = Ainv * B[:, :, n]
+    else:
+        bs = X0
+    for it in range(maxiter):
+        Ab = np.tensordot(bs, A(bs), axes=([0, 1], [0, 1]))
+        Bb = np.tensordot(bs, B, axes=([0, 1], [0, 1]))
+        Xb = np.linalg.solve(Ab, Bb)
+        X = np.tensordot(bs, Xb, axes=(2, 0))
+        W = A(X) - B
+        norms = np.array([_norm(W[:, :, n]) for n in range(k)])
+        if (norms < conv).all():
+            return X
+        dkmax = min(kmax - l, k)
+        eps = 0.01 * conv
+        dk = min(dkmax, int(np.sum(norms > eps)))
+        Qs = np.zeros((nocc, nvirt, dk))
+        nb = 0
+        for n in range(dkmax):
+            if norms[n] > eps:
+                Qs[:, :, nb] = Ainv * W[:, :, n]
+                nb += 1
+        bs = gram_schmidt_2d(np.concatenate([bs, Qs], axis=2))
+        l = bs.shape[-1]
+    raise Exception("Z-vector equations: Krylov solver for linear system did not converge after %s iterations "
+                    "(threshold: %e) !!!" % (it + 1, conv))
 
 
 def ct_shifts(q_oo, q_vv, gaussOmega, positions, orbe_virt, omega, Sc=0.0001, Rc=0.1):

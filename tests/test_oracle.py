@@ -299,3 +299,20 @@ def test_ct_shifts_separated_fragments():
     np.testing.assert_allclose(T[cross], closed, rtol=1e-6, atol=1e-14)
     assert T[cross].max() > 0.04                       # H2 sigma -> C8H10 LUMO is shifted
     assert np.all(T[~cross] == 0.0) and np.all(Ria[~cross] < 1e-4)
+
+
+def test_zvector_krylov_solves_dense_system(rng):
+    """Krylov_solver_Zvector (Solver.py:921-1012) with the reference's (A+B)
+    products reproduces the dense solution (A+B)^{-1} B of buildAB's matrices."""
+    gs = ground_state("C8H10", 1, 0.05, 2)
+    f, orbe, C, S, ao, av = _active(gs)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    omega, df = O.orbital_differences(orbe, f, ao, av)
+    g, glr = gs.getGamma(), gs.getGamma_lr()
+    o, v = omega.shape
+    A, B = O.buildAB(g, glr, q_oo, q_vv, q_ov, omega, 0 * omega, df, o, v)
+    rhs = rng.standard_normal((o, v, 2))
+    X = O.Krylov_solver_Zvector(lambda vs: O.ApBv(g, glr, q_oo, q_ov, q_vv, omega, vs, 1), omega, rhs, None,
+                                conv=1e-10)
+    Xd = np.linalg.solve(A + B, rhs.reshape(o * v, 2)).reshape(o, v, 2)
+    assert np.abs(X - Xd).max() < 1e-9 * np.abs(Xd).max()
