@@ -11,10 +11,11 @@ producer in ``dftbaby_amd.groundstate``.
 Every excited-state pass runs on the GPU: transition charges (kernel (a)), the
 Casida matvecs (b) and the Davidson loop (c), Oia / Lambda2 and the transition
 dipoles.  The long-range CT correction (ct_correction=1, lc = 0 only) shifts
-the diagonal as in LR_TDDFTB.py:429-437 (``dftbaby_amd.LongRangeCT``).  Out of
-scope (raise NotImplementedError): multipole charges (mulliken_dipoles=1),
-symmetry selection, the stability analysis and Tamm-Dancoff (which the
-reference itself disables, Solver.py:194).
+the diagonal as in LR_TDDFTB.py:429-437 (``dftbaby_amd.LongRangeCT``);
+stability=1 prints the lowest eigenvalues of the stability matrix
+(``dftbaby_amd.StabilityAnalysis``).  Out of scope (raise NotImplementedError):
+multipole charges (mulliken_dipoles=1), symmetry selection and Tamm-Dancoff
+(which the reference itself disables, Solver.py:194).
 """
 import time
 
@@ -174,6 +175,7 @@ class LR_TDDFTB:
     # ------------------------------------------------------------- core solve
     def _build_operator(self):
         t0 = time.time()
+        self._qref = None          # reference-layout charges belong to the previous geometry
         orbs = self.dftb2.getKSCoefficients()
         S = self.dftb2.getOverlapMatrix()
         orbe = self.dftb2.getKSEnergies()
@@ -227,8 +229,6 @@ class LR_TDDFTB:
                             diag_conv=1.0e-10, diag_maxiter=10, diag_check=1, diag_L2threshold=0.0,
                             diag_selected_irreps=None, ct_correction=0, stability=0, precision="fp64"):
         """LR_TDDFTB.py:403-527 on the GPU."""
-        if stability == 1:
-            raise NotImplementedError("the ground-state stability analysis is out of scope")
         if response_method == "Tamm-Dancoff":
             raise Exception("Revise the TDA code!!!!")   # Solver.py:194: disabled in the reference
         if response_method != "Casida":
@@ -237,6 +237,13 @@ class LR_TDDFTB:
         op = self.op
         lc = op.lc
         prec = precision_code(precision)
+        # stability of the ground state (LR_TDDFTB.py:423-427)
+        if stability == 1:
+            from .StabilityAnalysis import check_stability
+            g = self.dftb2.getGamma()
+            glr = self.dftb2.getGamma_lr() if lc == 1 else 0.0 * g
+            qoo, qov, qvv = self._charges_ref()
+            self.stability_eigenvalues = check_stability(g, glr, qoo, qvv, qov, self.omega, self.nocc, self.nvirt)
         # long-range CT correction: shifts of the orbital-energy differences (LR_TDDFTB.py:429-437)
         self.ct_correction = ct_correction
         omega_shift = None

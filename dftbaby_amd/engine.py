@@ -1114,3 +1114,36 @@ def zvector_gpu(op, B, X0=None, maxiter=1000, conv=1.0e-14, precision=0, verbose
         l = l_new
     raise Exception("Z-vector equations: Krylov solver for linear system did not converge after %s iterations "
                     "(threshold: %e) !!!" % (it + 1, conv))
+
+
+def stability_gpu(q_oo, q_vv, q_ov, omega, gamma, gamma_lr, nev=6, chunk=512):
+    """Lowest eigenvalues of the restricted -> unrestricted stability matrix
+    R = diag(omega) - (ij|ab) - (ia|jb)_lr (StabilityAnalysis.py:24-56).
+
+    R is the A block ((A+B) + (A-B)) / 2 of a Casida operator whose Coulomb
+    kernel is -gamma_lr / 2 and whose exchange kernel is gamma: its columns come
+    from the matvec kernel on unit vectors, its lowest nev eigenvalues from the
+    GPU eigensolver (the reference builds R densely and calls eigsh).  Charges in
+    the reference layout; n_ov <= 4096."""
+    op = CasidaOperatorGPU.from_charges(q_oo, q_vv, q_ov, omega, -0.5 * np.asarray(gamma_lr, dtype=np.float64),
+                                        gamma, lc=1)
+    dev = op.device
+    st = _stream()
+    o, v, N = op.nocc, op.nvirt, op.N
+    n = o * v
+    if n > 4096:
+        raise ValueError("the stability matrix is limited to n_ov <= 4096 (got %d)" % n)
+    real = (torch.arange(o, device=dev)[:, None] * op.vpad + torch.arange(v, device=dev)[None, :]).reshape(-1)
+    R = torch.empty((n, n), dtype=torch.float64, device=dev)
+    for c0 in range(0, n, chunk):
+        m = min(chunk, n - c0)
+        V = torch.empty((m, N), dtype=torch.float64, device=dev)
+        call("dftb_unit_vectors", _p(real[c0:c0 + m].contiguous()), m, N, _p(V), st)
+        P, M = op.matvec(V, 0)
+        R[c0:c0 + m] = 0.5 * (P[:, real] + M[:, real])
+    k = min(nev, n)
+    w = torch.empty(k, dtype=torch.float64, device=dev)
+    F = torch.empty((n, k), dtype=torch.float64, device=dev)
+    ws, wn = op.ws.get(_lib.load().dftb_syevx_workspace(n, k))
+    call("dftb_syevx", _p(R), n, n, k, _p(w), _p(F), _p(ws), wn, st)
+    return w.cpu().numpy()

@@ -27,6 +27,8 @@ Pinning (see tests/test_oracle.py):
                          independent dense evaluation; Mulliken variant from q.
   Oia / Lambda2        - brute-force double loop of LR_TDDFTB.py:538-560.
   Krylov_solver_Zvector- the dense solution (A+B)^{-1} B of buildAB's matrices.
+  check_stability      - R assembled element by element from its definition
+                         (loops over i, a, j, b) on a small molecule.
   ct_shifts            - separated fragments: S_ia -> 0 and R_ia -> the distance
                          between the centres of two centrosymmetric fragments,
                          so T_ia reduces to the closed form of the asymptotic
@@ -418,6 +420,17 @@ def Lambda2_calcOia_loop(q_oo, q_vv, gaussOmega):
             Oaa = q_vv[:, a, a] @ (gaussOmega @ q_vv[:, a, a])
             Oia[i, a] = q_oo[:, i, i] @ (gaussOmega @ q_vv[:, a, a]) / np.sqrt(Oii * Oaa)
     return Oia
+
+
+def check_stability(gamma, gamma_lr, qtrans_oo, qtrans_vv, qtrans_ov, omega, nocc, nvirt, k=6):
+    """Lowest eigenvalues of R = diag(omega) - (ij|ab) - (ia|jb)_lr,
+    StabilityAnalysis.py:24-56 (dense eigh instead of eigsh(which='SA'))."""
+    ijab = np.tensordot(qtrans_oo, np.tensordot(gamma, qtrans_vv, axes=(1, 0)), axes=(0, 0))
+    ijab = np.swapaxes(ijab, 1, 2)
+    iajb_lr = np.tensordot(qtrans_ov, np.tensordot(gamma_lr, qtrans_ov, axes=(1, 0)), axes=(0, 0))
+    n = nocc * nvirt
+    R = np.diag(np.reshape(omega, n)) - np.reshape(ijab, (n, n)) - np.reshape(iajb_lr, (n, n))
+    return np.linalg.eigvalsh(R)[:min(k, n)], R
 
 
 def gram_schmidt_2d(W):

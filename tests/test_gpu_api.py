@@ -180,3 +180,21 @@ def test_zvector_solver(rng):
     assert np.abs(Xw - Xd).max() < 1e-9 * np.abs(Xd).max()
     with pytest.raises(TypeError):
         Solver.Krylov_solver_Zvector(lambda x: x, omega, rhs, None)
+
+
+@pytest.mark.parametrize("lc", [0, 1])
+def test_stability_analysis(lc, capsys):
+    """stability=1 (LR_TDDFTB.py:423-427, StabilityAnalysis.py:24-56): the lowest
+    eigenvalues of R from the GPU (matvec columns, GPU eigensolver) equal the
+    oracle's dense ones, and the report is printed."""
+    td, gs = _td("C8H10", lc)
+    td.getEnergies(nstates=2, stability=1, diag_conv=1e-8, diag_maxiter=100)
+    f, orbe, C, S = gs.getOccupation(), gs.getKSEnergies(), gs.getKSCoefficients(), gs.getOverlapMatrix()
+    _, _, ao, av = O.active_orbitals(f, orbe)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    omega, _ = O.orbital_differences(orbe, f, ao, av)
+    g = gs.getGamma()
+    glr = gs.getGamma_lr() if lc else 0 * g
+    w, _ = O.check_stability(g, glr, q_oo, q_vv, q_ov, omega, *omega.shape)
+    np.testing.assert_allclose(td.stability_eigenvalues, w, atol=1e-10)
+    assert "lowest eigenvalue of restricted/unrestricted stability matrix R" in capsys.readouterr().out

@@ -316,3 +316,25 @@ def test_zvector_krylov_solves_dense_system(rng):
                                 conv=1e-10)
     Xd = np.linalg.solve(A + B, rhs.reshape(o * v, 2)).reshape(o, v, 2)
     assert np.abs(X - Xd).max() < 1e-9 * np.abs(Xd).max()
+
+
+def test_stability_matrix_elementwise():
+    """StabilityAnalysis.py:24-56: the stability matrix of the oracle equals its
+    element-wise definition R_{ia,jb} = w_ia d_ij d_ab - sum_AB q^ij_A g_AB q^ab_B
+    - sum_AB q^ia_A glr_AB q^jb_B on ethylene (lc = 1)."""
+    gs = ground_state("ethylene", 1, 0.05, 4)
+    f, orbe, C, S, ao, av = _active(gs)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    omega, _ = O.orbital_differences(orbe, f, ao, av)
+    g, glr = gs.getGamma(), gs.getGamma_lr()
+    o, v = omega.shape
+    w, R = O.check_stability(g, glr, q_oo, q_vv, q_ov, omega, o, v)
+    Rl = np.zeros((o, v, o, v))
+    for i in range(o):
+        for a in range(v):
+            for j in range(o):
+                for b in range(v):
+                    Rl[i, a, j, b] = ((omega[i, a] if (i == j and a == b) else 0.0)
+                                      - q_oo[:, i, j] @ g @ q_vv[:, a, b] - q_ov[:, i, a] @ glr @ q_ov[:, j, b])
+    assert np.abs(R - Rl.reshape(o * v, o * v)).max() < 1e-13
+    np.testing.assert_allclose(w, np.linalg.eigvalsh(Rl.reshape(o * v, o * v))[:len(w)], atol=1e-13)
