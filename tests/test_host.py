@@ -14,3 +14,13 @@ def test_initial_indices_match_stable_argsort():
         lmax = int(rng.integers(1, o * v + 1))
         i, a = _initial_indices(vals, o, v, lmax)
         assert np.array_equal(i * v + a, np.argsort(vals.ravel(), kind="stable")[:lmax])
+
+
+def test_zvector_solver_needs_the_gpu_operator():
+    """Solver.Krylov_solver_Zvector takes the (A+B) operator object, not a Python
+    callable (the products run on the GPU); the check precedes any device work."""
+    import numpy as np
+    import pytest
+    from dftbaby_amd import Solver
+    with pytest.raises(TypeError):
+        Solver.Krylov_solver_Zvector(lambda x: x, np.ones((2, 3)), np.ones((2, 3, 1)), None)
