@@ -173,7 +173,9 @@ class LR_TDDFTB:
         return self.qtrans_oo, self.qtrans_vv, self.qtrans_ov
 
     # ------------------------------------------------------------- core solve
-    def _build_operator(self):
+    def _build_operator(self, coulomb=True):
+        """Operator of the active space; coulomb=False drops the Coulomb kernel (the
+        triplet matrices of the dense Casida, Solver.py:71-80: K_singlet only for "S")."""
         t0 = time.time()
         self._qref = None          # reference-layout charges belong to the previous geometry
         orbs = self.dftb2.getKSCoefficients()
@@ -182,6 +184,8 @@ class LR_TDDFTB:
         ao, av = self.active_occupied_orbs, self.active_virtual_orbs
         self.omega = orbe[av][None, :] - orbe[ao][:, None]
         gamma = self.dftb2.getGamma()
+        if not coulomb:
+            gamma = 0.0 * np.asarray(gamma)
         lc = int(getattr(self.dftb2, "long_range_correction", 0))
         gamma_lr = self.dftb2.getGamma_lr() if lc == 1 else 0.0 * gamma
         if getattr(self.dftb2, "mulliken_dipoles", 0) == 1:
@@ -233,7 +237,9 @@ class LR_TDDFTB:
             raise Exception("Revise the TDA code!!!!")   # Solver.py:194: disabled in the reference
         if response_method != "Casida":
             raise Exception("response_method should be \"Casida\" or \"Tamm-Dancoff\"")
-        self._build_operator()
+        # the dense Casida honours the multiplicity (triplets: no Coulomb kernel); the
+        # Davidsons compute singlets, as in the reference (LR_TDDFTB.py:471-492)
+        self._build_operator(coulomb=not (nstates is None and multiplicity == "T"))
         op = self.op
         lc = op.lc
         prec = precision_code(precision)

@@ -198,3 +198,23 @@ def test_stability_analysis(lc, capsys):
     w, _ = O.check_stability(g, glr, q_oo, q_vv, q_ov, omega, *omega.shape)
     np.testing.assert_allclose(td.stability_eigenvalues, w, atol=1e-10)
     assert "lowest eigenvalue of restricted/unrestricted stability matrix R" in capsys.readouterr().out
+
+
+@pytest.mark.parametrize("lc", [0, 1])
+def test_dense_casida_triplets(lc):
+    """nstates=None with multiplicity="T": the dense Casida of the triplet matrices
+    (no Coulomb kernel, Solver.py:71-80) as the oracle's, oscillator strengths 0."""
+    td, gs = _td("C8H10", lc)
+    Omega, osc = td.getEnergies(nstates=None, multiplicity="T")
+    f, orbe, C, S = gs.getOccupation(), gs.getKSEnergies(), gs.getKSCoefficients(), gs.getOverlapMatrix()
+    _, _, ao, av = O.active_orbitals(f, orbe)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    omega, df = O.orbital_differences(orbe, f, ao, av)
+    g = gs.getGamma()
+    glr = gs.getGamma_lr() if lc else 0 * g
+    o, v = omega.shape
+    ref = O.Casida(g, glr, q_oo, q_vv, q_ov, omega, 0 * omega, df, o, v, multiplicity="T")[0]
+    refS = O.Casida(g, glr, q_oo, q_vv, q_ov, omega, 0 * omega, df, o, v, multiplicity="S")[0]
+    np.testing.assert_allclose(Omega, ref, atol=1e-10)
+    assert np.abs(ref[:5] - refS[:5]).max() > 1e-3
+    assert np.all(osc == 0.0)
