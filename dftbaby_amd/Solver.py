@@ -7,14 +7,14 @@ out), computed on the GPU by ``dftbaby_amd.engine``.  The extra keyword
 (A+-B) products; the Davidson residual, orthonormalisation and subspace
 eigensolve are FP64 in every mode.
 
-Reference: HermitianDavidson Solver.py:404-578, nonHermitianDavidson
-Solver.py:637-917, Krylov_solver_Zvector Solver.py:921-1012, exceptions
-Solver.py:27-33.
+Reference: Casida Solver.py:98-181, HermitianDavidson Solver.py:404-578,
+nonHermitianDavidson Solver.py:637-917, Krylov_solver_Zvector
+Solver.py:921-1012, exceptions Solver.py:27-33.
 """
 import numpy as np
 
 from .engine import (CasidaOperatorGPU, ExcitedStatesError, ExcitedStatesNotConverged,  # noqa: F401
-                     hermitian_davidson_gpu, nonhermitian_davidson_gpu, zvector_gpu)
+                     casida_dense_gpu, hermitian_davidson_gpu, nonhermitian_davidson_gpu, zvector_gpu)
 
 _PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, 0: 0, 1: 1, 2: 2}
 
@@ -27,6 +27,27 @@ def precision_code(precision):
 
 def _out(op, tensors):
     return tuple(op.from_padded(t).cpu().numpy().copy() for t in tensors)
+
+
+def Casida(gamma, gamma_lr, qtrans_oo, qtrans_vv, qtrans_ov, omega, omega_shift, df, nocc, nvirt, multiplicity="S"):
+    """Full Casida solve for all nocc*nvirt roots (Solver.py:98-181), n_ov <= 4096:
+    (A+-B) assembled from the GPU products, the (A-B)^{1/2}(A+B)(A-B)^{1/2} problem
+    solved by the GPU eigensolver.  Triplets drop the Coulomb kernel (71-80); the CT
+    shift enters the diagonals of A and B (46-49).  Returns (Omega, C, XmY, XpY)
+    with the vectors as (nocc*nvirt, nocc, nvirt).  Closed-shell occupations only
+    (df = 2 on every pair)."""
+    if not np.allclose(np.asarray(df, dtype=np.float64), 2.0):
+        raise NotImplementedError("Casida: closed-shell occupation differences (df = 2) only")
+    assert multiplicity in ("S", "T")
+    g = np.asarray(gamma, dtype=np.float64)
+    glr = np.asarray(gamma_lr, dtype=np.float64)
+    lc = int(np.any(glr != 0.0))
+    op = CasidaOperatorGPU.from_charges(qtrans_oo, qtrans_vv, qtrans_ov, omega, g if multiplicity == "S" else 0.0 * g,
+                                        glr, lc=lc)
+    shift = None if omega_shift is None or not np.any(omega_shift) else omega_shift
+    w, C, XmY, XpY = casida_dense_gpu(op, omega_shift=shift)
+    C, XmY, XpY = _out(op, (C, XmY, XpY))
+    return w.cpu().numpy(), C, XmY, XpY
 
 
 def HermitianDavidson(g0, q_ov, omega, omega_shift, nocc, nvirt, XmYguess, XpYguess, Oia, selector=None,

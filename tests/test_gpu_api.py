@@ -218,3 +218,24 @@ def test_dense_casida_triplets(lc):
     np.testing.assert_allclose(Omega, ref, atol=1e-10)
     assert np.abs(ref[:5] - refS[:5]).max() > 1e-3
     assert np.all(osc == 0.0)
+
+
+@pytest.mark.parametrize("mult", ["S", "T"])
+def test_solver_casida(mult):
+    """Solver.Casida (Solver.py:98-181) on the GPU against the oracle: all roots,
+    X-Y and X+Y up to sign, reference layouts (n_ov, nocc, nvirt)."""
+    from dftbaby_amd import Solver
+    gs = ground_state("C8H10", 1, 0.05, 13)
+    f, orbe, C, S = gs.getOccupation(), gs.getKSEnergies(), gs.getKSCoefficients(), gs.getOverlapMatrix()
+    _, _, ao, av = O.active_orbitals(f, orbe)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    omega, df = O.orbital_differences(orbe, f, ao, av)
+    g, glr = gs.getGamma(), gs.getGamma_lr()
+    o, v = omega.shape
+    Om, Cg, XmY, XpY = Solver.Casida(g, glr, q_oo, q_vv, q_ov, omega, 0 * omega, df, o, v, multiplicity=mult)
+    ref = O.Casida(g, glr, q_oo, q_vv, q_ov, omega, 0 * omega, df, o, v, multiplicity=mult)
+    assert XpY.shape == (o * v, o, v)
+    np.testing.assert_allclose(Om, ref[0], atol=1e-10)
+    for n in range(8):
+        for a, b in ((XmY[n], ref[2][n]), (XpY[n], ref[3][n])):
+            assert min(np.abs(a - b).max(), np.abs(a + b).max()) < 1e-8
