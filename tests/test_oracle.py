@@ -338,3 +338,24 @@ def test_stability_matrix_elementwise():
                                       - q_oo[:, i, j] @ g @ q_vv[:, a, b] - q_ov[:, i, a] @ glr @ q_ov[:, j, b])
     assert np.abs(R - Rl.reshape(o * v, o * v)).max() < 1e-13
     np.testing.assert_allclose(w, np.linalg.eigvalsh(Rl.reshape(o * v, o * v))[:len(w)], atol=1e-13)
+
+
+def test_hermitian_davidson_with_ct_shift_matches_dense():
+    """HermitianDavidson with omega_shift (Solver.py:404-578: diagonal w^2 + 2 w T)
+    finds the lowest roots of the dense Casida with the same shift (Solver.py:46-49:
+    T on the diagonals of A and B), and the shift moves them."""
+    gs = ground_state("C8H10-H2", 0)
+    f, orbe, C, S, ao, av = _active(gs)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    omega, df = O.orbital_differences(orbe, f, ao, av)
+    pos = np.array([p for _, p in gs.getGeometry()])
+    T, _, _ = O.ct_shifts(q_oo, q_vv, gs.getGaussianOverlap(), pos, orbe[av], omega)
+    g = gs.getGamma()
+    o, v = omega.shape
+    Oia = O.Lambda2_calcOia(q_oo, q_vv, gs.getGaussianOverlap())
+    dense = O.Casida(g, 0 * g, q_oo, q_vv, q_ov, omega, T, df, o, v)[0]
+    dense0 = O.Casida(g, 0 * g, q_oo, q_vv, q_ov, omega, 0 * T, df, o, v)[0]
+    w = O.HermitianDavidson(g, q_ov, omega, T, o, v, None, None, Oia, nstates=14, ifact=4, conv=1e-11,
+                            maxiter=200, L2_thresh=0.0)[0]
+    np.testing.assert_allclose(w, dense[:14], atol=1e-10)
+    assert np.abs(dense[:14] - dense0[:14]).max() > 0.01
