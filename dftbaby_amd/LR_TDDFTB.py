@@ -270,7 +270,6 @@ class LR_TDDFTB:
                 XpYguess=self.XpY, Oia=self.Oia, L2_thresh=diag_L2threshold, precision=prec, verbose=verbose)
             self.davidson_stats = st
         else:
-            assert multiplicity == "S"
             res = None
             if self.guess_model is not None and self.XpY is None:
                 from .learned import learned_start

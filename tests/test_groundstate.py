@@ -50,3 +50,91 @@ def test_scc_orthonormal_orbitals_and_neutral_charges():
     f = gs.getOccupation()
     assert set(np.round(f, 12)) <= {0.0, 2.0}
     assert np.sum(f) == sum(4 if z == 6 else 1 for z, _ in gs.getGeometry())
+
+
+def _coulomb_gaussians_quadrature(R, s2, Rlr=None):
+    """Interaction energy of two normalised spherical Gaussian charge clouds
+    F(r) = (2 pi sigma^2)^{-3/2} exp(-r^2 / (2 sigma^2)) whose squared widths sum
+    to s2, at distance R, through 1/r (or the long-range erf(r / Rlr) / r of
+    GammaApproximation.py:210-243), by 1-D quadrature of the Fourier
+    representation: E = (2/pi) int_0^inf exp(-s2 k^2 / 2) [exp(-k^2 Rlr^2 / 4)]
+    sin(kR) / (kR) dk (each cloud transforms to exp(-sigma^2 k^2 / 2), 1/r to
+    4 pi / k^2, erf(r/Rlr)/r to 4 pi exp(-k^2 Rlr^2 / 4) / k^2)."""
+    from scipy.integrate import quad
+    lr = 0.0 if Rlr is None else 0.25 * Rlr ** 2
+    f = lambda k: np.exp(-(0.5 * s2 + lr) * k * k) * (np.sinc(k * R / np.pi) if R > 0 else 1.0)
+    val, err = quad(f, 0.0, np.inf, epsabs=1e-14, epsrel=1e-13, limit=400)
+    return 2.0 / np.pi * val
+
+
+def _overlap_gaussians_quadrature(R, s2):
+    """Omega_AB = int F_A F_B d^3r (DFTB2.py:1287-1317) by 1-D quadrature of
+    (2 pi^2)^{-1} int_0^inf k^2 exp(-s2 k^2 / 2) sin(kR) / (kR) dk."""
+    from scipy.integrate import quad
+    f = lambda k: k * k * np.exp(-0.5 * s2 * k * k) * (np.sinc(k * R / np.pi) if R > 0 else 1.0)
+    kmax = np.sqrt(2.0 * 50.0 / s2)          # the integrand is below exp(-50) beyond
+    val, err = quad(f, 0.0, kmax, epsabs=1e-15, epsrel=1e-12, limit=1000)
+    return val / (2.0 * np.pi ** 2)
+
+
+def test_gamma_bonded_pairs_quadrature():
+    """gamma_AB and gamma^lr_AB at bonding distances (C=C, C-C, C-H of the polyene
+    and a C...C next-nearest pair), where erf(C_AB R) is far from 1, against the
+    quadrature of two Gaussian clouds with sigma = 1 / (sqrt(pi) U)
+    (GammaApproximation.py:122-151, 210-243).  Planted mistakes in C_AB (the
+    factor 2 dropped; R_lr^2 instead of R_lr^2 / 2) fail the same comparison."""
+    from scipy.special import erf
+    gs = ground_state("C8H10", 1, 0.05, 17)
+    g, glr = gs.getGamma(), gs.getGamma_lr()
+    Z = np.array([z for z, _ in gs.getGeometry()])
+    pos = np.array([p for _, p in gs.getGeometry()])
+    U = np.array([gs.hubbard_U[z] for z in Z])
+    sig2 = 1.0 / (np.pi * U ** 2)
+    Rlr = gs.long_range_radius
+    D = np.linalg.norm(pos[:, None] - pos[None, :], axis=-1)
+    checked = 0
+    for A in range(len(Z)):
+        for B in range(A + 1, len(Z)):
+            if D[A, B] > 5.0:
+                continue
+            s2 = sig2[A] + sig2[B]
+            gq = _coulomb_gaussians_quadrature(D[A, B], s2)
+            gq_lr = _coulomb_gaussians_quadrature(D[A, B], s2, Rlr)
+            assert abs(g[A, B] - gq) < 1e-11, (A, B, g[A, B], gq)
+            assert abs(glr[A, B] - gq_lr) < 1e-11, (A, B, glr[A, B], gq_lr)
+            wrong_lr = erf(D[A, B] / np.sqrt(2 * (s2 + Rlr ** 2))) / D[A, B]
+            assert abs(wrong_lr - gq_lr) > 1e-3
+            if D[A, B] < 3.0:                     # bonds: erf(C_AB R) well below 1
+                assert gq < 0.99 / D[A, B]
+                wrong = erf(D[A, B] / np.sqrt(s2)) / D[A, B]
+                assert abs(wrong - gq) > 1e-3
+                checked += 1
+    assert checked >= 15
+    # the on-site limits from the same quadrature at R = 0
+    for A in (0, len(Z) - 1):
+        assert abs(g[A, A] - _coulomb_gaussians_quadrature(0.0, 2 * sig2[A])) < 1e-11
+        assert abs(glr[A, A] - _coulomb_gaussians_quadrature(0.0, 2 * sig2[A], Rlr)) < 1e-11
+
+
+def test_gaussian_overlap_quadrature():
+    """Omega_AB (DFTB2.py:1287-1317, sigma_A = 1.329 / sqrt(8 ln 2) / U_A) against
+    the quadrature of the overlap integral of the two normalised clouds, on-site
+    and for bonded and distant pairs; planted mistake: sigma = 1 / (sqrt(pi) U)
+    (the gamma widths) fails."""
+    gs = ground_state("C8H10", 1, 0.05, 17)
+    Om = gs.getGaussianOverlap()
+    Z = np.array([z for z, _ in gs.getGeometry()])
+    pos = np.array([p for _, p in gs.getGeometry()])
+    U = np.array([gs.hubbard_U[z] for z in Z])
+    sig = 1.329 / np.sqrt(8.0 * np.log(2.0)) / U
+    D = np.linalg.norm(pos[:, None] - pos[None, :], axis=-1)
+    rng = np.random.default_rng(3)
+    pairs = [(0, 0), (len(Z) - 1, len(Z) - 1)] + [tuple(p) for p in rng.integers(0, len(Z), size=(25, 2))]
+    for A, B in pairs:
+        s2 = sig[A] ** 2 + sig[B] ** 2
+        ref = _overlap_gaussians_quadrature(D[A, B], s2)
+        assert abs(Om[A, B] - ref) <= 1e-10 * max(ref, 1e-3), (A, B, Om[A, B], ref)
+        sw2 = 1.0 / (np.pi * U[A] ** 2) + 1.0 / (np.pi * U[B] ** 2)
+        if D[A, B] < 4.0:
+            assert abs(_overlap_gaussians_quadrature(D[A, B], sw2) - ref) > 1e-4 * ref
+    np.testing.assert_allclose(Om, Om.T, rtol=0, atol=0)

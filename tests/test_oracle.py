@@ -107,10 +107,16 @@ def test_trans_charges_invariants_and_bruteforce():
     ve = O.trans_charges(C[:, sub_o], C[:, sub_v], S, gs.orbsPerAtom)
     for a, b in zip(bf, ve):
         np.testing.assert_allclose(a, b, atol=1e-14)
-    # a planted mistake (dropping the symmetrising term) is caught by the invariant
-    Sc = S @ C[:, av]
-    wrong = np.einsum("mi,ma->ia", C[:, av], Sc)  # no per-atom split: equals identity, but per-atom halves differ
-    assert wrong.shape == (len(av), len(av))
+    # planted mistake: per-atom charges without the symmetrising term,
+    # q^{ab}_A = sum_{mu in A} C_{mu a} (SC)_{mu b}.  They still sum to C^T S C = 1
+    # over the atoms, so only the symmetry and brute-force checks can catch them.
+    Cv, Scv = C[:, sub_v], (S @ C)[:, sub_v]
+    amap = np.repeat(np.arange(len(gs.orbsPerAtom)), gs.orbsPerAtom)
+    wrong = np.zeros_like(bf[2])
+    np.add.at(wrong, amap, Cv[:, :, None] * Scv[:, None, :])
+    np.testing.assert_allclose(wrong.sum(0), np.eye(len(sub_v)), atol=1e-12)
+    assert np.abs(wrong - wrong.transpose(0, 2, 1)).max() > 1e-4
+    assert np.abs(wrong - bf[2]).max() > 1e-4
 
 
 # --------------------------------------------------------------------------- a7
@@ -359,3 +365,74 @@ def test_hermitian_davidson_with_ct_shift_matches_dense():
                             maxiter=200, L2_thresh=0.0)[0]
     np.testing.assert_allclose(w, dense[:14], atol=1e-10)
     assert np.abs(dense[:14] - dense0[:14]).max() > 0.01
+
+
+# --------------------------------------------------------------------------- a9
+def test_dipoles_equal_mulliken_on_point_charge_dipole_matrix(rng):
+    """With the dipole matrix D^c = S o (R^c_mu + R^c_nu) / 2 of AO centres,
+    <i|r_c|a> = sum_A R^c_A q^{ia}_A exactly (expand q^{ia}_A = 1/2 sum_{mu in A}
+    [C_mu,i (SC)_mu,a + C_mu,a (SC)_mu,i] and swap mu, nu in the second term), so
+    the "Dipoles" (LR_TDDFTB.py:574-592) and "Mulliken" (567-573) transition dipoles
+    of the oracle must agree for any coefficients C^I_ia.  Planted mistake: the
+    unsymmetrised D = S o R_mu breaks the equality."""
+    gs = ground_state("C8H10", 1, 0.05, 21)
+    f, orbe, C, S, ao, av = _active(gs)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    pos = np.array([p for _, p in gs.getGeometry()])
+    Rmu = pos[np.repeat(np.arange(len(pos)), gs.orbsPerAtom)]
+    D = S[:, :, None] * 0.5 * (Rmu[:, None, :] + Rmu[None, :, :])
+    Cij = rng.standard_normal((5, len(ao), len(av)))
+    dip = O.transition_dipoles(Cij, C, D, ao, av)
+    mul = O.transition_dipoles_mulliken(Cij, q_ov, pos)
+    scale = max(np.abs(d).max() for d in dip)
+    for c in range(3):
+        assert np.abs(dip[c] - mul[c]).max() < 1e-12 * scale
+    wrongD = S[:, :, None] * Rmu[:, None, :]
+    wrong = O.transition_dipoles(Cij, C, wrongD, ao, av)
+    assert max(np.abs(wrong[c] - mul[c]).max() for c in range(3)) > 1e-3 * scale
+
+
+# --------------------------------------------------------------------------- a1
+def _active_bruteforce(f, orbe, nocc_act, nvirt_act):
+    """Top-k by pairwise comparison: occupied p (f > 0.1) is active when fewer than
+    nocc_act occupied orbitals rank above it in (energy, index); virtual a when
+    fewer than nvirt_act rank below it.  Both sets ascending in (energy, index)."""
+    n = len(f)
+    occ = [p for p in range(n) if f[p] > 0.1]
+    virt = [p for p in range(n) if f[p] <= 0.1]
+    key = lambda p: (orbe[p], p)
+    act_o = [p for p in occ if nocc_act is None or sum(key(q) > key(p) for q in occ) < nocc_act]
+    act_v = [p for p in virt if nvirt_act is None or sum(key(q) < key(p) for q in virt) < nvirt_act]
+    return np.array(sorted(act_o, key=key), dtype=int), np.array(sorted(act_v, key=key), dtype=int)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_active_orbitals_truncation_bruteforce(seed):
+    """setActiveOrbitals (LR_TDDFTB.py:167-188): the highest nr_active_occ occupied
+    and lowest nr_active_virt virtual orbitals, each ascending in energy, on
+    shuffled energies with ties and fractional occupations; the oracle and the
+    drop-in's host selection against the pairwise brute force (index work:
+    exact)."""
+    from dftbaby_amd.LR_TDDFTB import LR_TDDFTB
+    r = np.random.default_rng(seed)
+    n = 23
+    orbe = np.round(r.standard_normal(n), 1)          # rounding makes ties
+    f = r.choice([0.0, 0.05, 0.1, 0.5, 1.0, 2.0], size=n)
+    for no, nv in [(None, None), (3, 4), (1, 1), (50, 2), (0, 5)]:
+        ref_o, ref_v = _active_bruteforce(f, orbe, no, nv)
+        _, _, ao, av = O.active_orbitals(f, orbe, no, nv)
+        np.testing.assert_array_equal(ao, ref_o)
+        np.testing.assert_array_equal(av, ref_v)
+
+        class GS:
+            getOccupation = staticmethod(lambda: f)
+            getKSEnergies = staticmethod(lambda: orbe)
+        td = LR_TDDFTB(ground_state=GS())
+        td.setActiveOrbitals(no, nv)
+        np.testing.assert_array_equal(td.active_occupied_orbs, ref_o)
+        np.testing.assert_array_equal(td.active_virtual_orbs, ref_v)
+        assert td.nocc == len(ref_o) and td.nvirt == len(ref_v)
+    # planted mistake: the lowest instead of the highest occupied orbitals
+    occ = np.where(f > 0.1)[0]
+    wrong = occ[np.argsort(orbe[occ], kind="stable")[:3]]
+    assert not np.array_equal(wrong, _active_bruteforce(f, orbe, 3, 4)[0])
