@@ -71,6 +71,8 @@ _SIGS = {
     "dftb_trans_charges_workspace": (_sz, [_i, _i, _i, _i]),
     "dftb_trans_charges": (_i, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _i, _i, _i, _i,
                                 _vp, _vp, _vp, _vp, _sz, _vp]),
+    "dftb_trans_charges_noalloc": (_i, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _i, _i, _i, _i,
+                                        _vp, _vp, _vp, _vp]),
     "dftb_trans_charges_engine": (_i, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _i, _i, _i, _i,
                                        _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "dftb_casida_matvec_workspace": (_sz, [ctypes.POINTER(CasidaOperator), _i]),

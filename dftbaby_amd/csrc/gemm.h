@@ -2,7 +2,9 @@
 #include "common.h"
 #include "../../include/dftbaby_amd.h"
 
-int gemm_launch(const GemmArgs& p, hipStream_t st);
+// timed = false keeps the launch out of the optional launch-timing record (which
+// creates HIP events): the allocation-free entry points use it.
+int gemm_launch(const GemmArgs& p, hipStream_t st, bool timed = true);
 
 // Optional launch timing (see dftb_gemm_timing_*): kernel classes
 enum { GT_F64_128 = 0, GT_F64_64 = 1, GT_F32_128 = 2, GT_F32_64 = 3, GT_BF3_PIPE = 4, GT_BF3_TILE = 5,

@@ -269,7 +269,7 @@ extern "C" int dftb_gemm_timing_read(int cls, double* ms_total, long* launches, 
   return DFTB_OK;
 }
 
-int gemm_launch(const GemmArgs& p0, hipStream_t st) {
+int gemm_launch(const GemmArgs& p0, hipStream_t st, bool timed) {
   GemmArgs p = p0;
   if (p.M <= 0 || p.N <= 0 || p.batch <= 0) return DFTB_OK;
   if (p.K < 0 || !p.A || !p.B || !p.C) return DFTB_ERR_ARG;
@@ -283,7 +283,8 @@ int gemm_launch(const GemmArgs& p0, hipStream_t st) {
   if (p.Ct) return DFTB_ERR_ARG;  // transposed copy: pipelined BF16x3 kernel only
   if (p.in_type != GEMM_IN_F64 && p.in_type != GEMM_IN_F32) return DFTB_ERR_ARG;
   GemmTimer tm;
-  tm.begin(st);
+  tm.slot = -1;
+  if (timed) tm.begin(st);
   if (p.in_type == GEMM_IN_F64) {
     // tall-skinny products of the Davidson basis (one side 20-40 vectors, the other
     // hundreds of basis vectors or the pair index): narrow tiles cut the padded

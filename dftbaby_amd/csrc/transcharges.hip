@@ -7,6 +7,7 @@
 // X and Y for those p are staged in LDS once, then every thread streams over
 // q reading X_{mu q}, Y_{mu q} (coalesced, reused PB times) and writes the PB
 // outputs q^{p q}_A for its q (coalesced along q).
+#include <mutex>
 #include <vector>
 #include "common.h"
 #include "gemm.h"
@@ -89,7 +90,8 @@ extern "C" size_t dftb_trans_charges_workspace(int norb, int nocc, int nvirt, in
 // Prepare S C products and atom offsets (orbs_per_atom is a host array, as the
 // f2py argument it replaces).
 static int tc_prepare(TcPlan& t, const double* Co, const double* Cv, const double* S, const int* orbs_per_atom,
-                      int nat, int norb, int nocc, int nvirt, void* ws, size_t ws_bytes, hipStream_t st) {
+                      int nat, int norb, int nocc, int nvirt, void* ws, size_t ws_bytes, hipStream_t st,
+                      bool timed = true) {
   if (nat <= 0 || norb <= 0 || nocc < 0 || nvirt < 0 || !S || !orbs_per_atom || !ws) return DFTB_ERR_ARG;
   if (ws_bytes < dftb_trans_charges_workspace(norb, nocc, nvirt, nat)) return DFTB_ERR_WORKSPACE;
   std::vector<int> off(nat + 1, 0);
@@ -117,24 +119,91 @@ static int tc_prepare(TcPlan& t, const double* Co, const double* Cv, const doubl
     g.B = pass == 0 ? Co : Cv; g.ldb = n;
     g.C = pass == 0 ? t.Yo : t.Yv; g.ldc = n;
     g.M = norb; g.N = n; g.K = norb;
-    int rc = gemm_launch(g, st);
+    int rc = gemm_launch(g, st, timed);
     if (rc) return rc;
   }
   return DFTB_OK;
 }
 
-extern "C" int dftb_trans_charges(const double* orbs_occ, const double* orbs_virt, const double* S,
-                                  const int* orbs_per_atom, int nat, int norb, int nocc, int nvirt, double* q_oo,
-                                  double* q_ov, double* q_vv, void* workspace, size_t ws_bytes, void* stream) {
-  hipStream_t st = (hipStream_t)stream;
+static int trans_charges_ref_layout(const double* orbs_occ, const double* orbs_virt, const double* S,
+                                    const int* orbs_per_atom, int nat, int norb, int nocc, int nvirt, double* q_oo,
+                                    double* q_ov, double* q_vv, void* workspace, size_t ws_bytes, hipStream_t st,
+                                    bool timed) {
   TcPlan t;
-  int rc = tc_prepare(t, orbs_occ, orbs_virt, S, orbs_per_atom, nat, norb, nocc, nvirt, workspace, ws_bytes, st);
+  int rc = tc_prepare(t, orbs_occ, orbs_virt, S, orbs_per_atom, nat, norb, nocc, nvirt, workspace, ws_bytes, st,
+                      timed);
   if (rc) return rc;
   long o = nocc, v = nvirt;
   if ((rc = tc_block(t, true, true, nocc, nocc, q_oo, o, o * o, st))) return rc;
   if ((rc = tc_block(t, true, false, nocc, nvirt, q_ov, v, o * v, st))) return rc;
   if ((rc = tc_block(t, false, false, nvirt, nvirt, q_vv, v, v * v, st))) return rc;
   return DFTB_OK;
+}
+
+extern "C" int dftb_trans_charges(const double* orbs_occ, const double* orbs_virt, const double* S,
+                                  const int* orbs_per_atom, int nat, int norb, int nocc, int nvirt, double* q_oo,
+                                  double* q_ov, double* q_vv, void* workspace, size_t ws_bytes, void* stream) {
+  return trans_charges_ref_layout(orbs_occ, orbs_virt, S, orbs_per_atom, nat, norb, nocc, nvirt, q_oo, q_ov, q_vv,
+                                  workspace, ws_bytes, (hipStream_t)stream, true);
+}
+
+// ---- allocation-free variant -------------------------------------------------
+// dftb_trans_charges_noalloc keeps its scratch (the S C products, norb * (nocc +
+// nvirt) doubles, and the atom offsets, nat + 1 ints) in this fixed device array
+// instead of a caller workspace.  Each device holds its own copy for the life of
+// the process.  Host code never passes `scratch` itself to a kernel or copy (on
+// the host the name is only a stand-in): it uses the device address that
+// hipGetSymbolAddress returns, looked up once per device on that device's first
+// call.  Every output element is overwritten, and the scratch contents of an
+// earlier call are never read, so nothing needs clearing between calls.  Calls
+// that overlap on one device (two host threads, or two streams without ordering
+// between them) share the array and race.
+#define TC_SCRATCH_BYTES ((size_t)64 << 20)
+#define TC_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[TC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[TC_MAX_DEVICES];
+
+static int scratch_address(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= TC_MAX_DEVICES) {
+    fprintf(stderr, "dftbaby_amd: dftb_trans_charges_noalloc: no usable current device (%d)\n", dev);
+    return DFTB_ERR_LAUNCH;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "dftbaby_amd: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev,
+              hipGetErrorString(e));
+      return DFTB_ERR_LAUNCH;
+    }
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return DFTB_OK;
+}
+
+extern "C" int dftb_trans_charges_noalloc(const double* orbs_occ, const double* orbs_virt, const double* S,
+                                          const int* orbs_per_atom, int nat, int norb, int nocc, int nvirt,
+                                          double* q_oo, double* q_ov, double* q_vv, void* stream) {
+  if (nat <= 0 || norb <= 0 || nocc < 0 || nvirt < 0) return DFTB_ERR_ARG;
+  const size_t need = dftb_trans_charges_workspace(norb, nocc, nvirt, nat);
+  if (need > TC_SCRATCH_BYTES) {
+    fprintf(stderr,
+            "dftbaby_amd: dftb_trans_charges_noalloc: norb = %d, nocc + nvirt = %d, nat = %d need %zu bytes of "
+            "scratch, above the fixed %zu (norb * (nocc + nvirt) <= %zu doubles); use dftb_trans_charges with a "
+            "workspace\n",
+            norb, nocc + nvirt, nat, need, TC_SCRATCH_BYTES, (TC_SCRATCH_BYTES - 65536) / sizeof(double));
+    return DFTB_ERR_WORKSPACE;
+  }
+  void* ws = nullptr;
+  int rc = scratch_address(&ws);
+  if (rc) return rc;
+  return trans_charges_ref_layout(orbs_occ, orbs_virt, S, orbs_per_atom, nat, norb, nocc, nvirt, q_oo, q_ov, q_vv,
+                                  ws, TC_SCRATCH_BYTES, (hipStream_t)stream, false);
 }
 
 static inline int pad8(int n) { return (n + 7) & ~7; }

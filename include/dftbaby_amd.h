@@ -96,6 +96,21 @@ int dftb_trans_charges(const double* orbs_occ, const double* orbs_virt, const do
                        double* q_oo, double* q_ov, double* q_vv,
                        void* workspace, size_t ws_bytes, void* stream);
 
+/* Allocation-free variant of dftb_trans_charges: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * S C products and atom offsets live in a fixed 64 MiB device array that the
+ * library holds on each device for the life of the process, so the call takes
+ * no workspace and calls no hipMalloc/hipFree.  Same arguments, layouts and
+ * results as dftb_trans_charges.  Limit: dftb_trans_charges_workspace(norb,
+ * nocc, nvirt, nat) <= 64 MiB (about norb * (nocc + nvirt) <= 8.38e6 doubles,
+ * e.g. norb = nocc + nvirt <= 2896); larger calls return DFTB_ERR_WORKSPACE (4)
+ * before any device work.  Calls on one device share the array: calls that
+ * overlap there (other host threads, unordered streams) race. */
+int dftb_trans_charges_noalloc(const double* orbs_occ, const double* orbs_virt, const double* S,
+                               const int* orbs_per_atom, int nat, int norb, int nocc, int nvirt,
+                               double* q_oo, double* q_ov, double* q_vv, void* stream);
+
 /* Same charges written in the engine's padded layouts (NULL = skip):
  *   q_oo_e [nocc][nat][opad], q_ov_e [nocc][nat][vpad], q_vv_e [vpad][nat][vpad],
  *   q_c    [nat][opad][vpad]  (the Coulomb layout).

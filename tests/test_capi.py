@@ -71,3 +71,22 @@ def test_gemm_rejects_invalid_transposed_copy_and_hadamard_arguments():
              args(Ct=fake, Ct2=fake, ldct=64, sCt=4096, in_type=_lib.GEMM_IN_F64, b_trans=1)]  # FP64 kernel
     for g in cases:
         assert lib.dftb_gemm(ctypes.byref(g), None) == 1
+
+
+def test_noalloc_trans_charges_rejects_sizes_beyond_its_scratch():
+    """dftb_trans_charges_noalloc checks its arguments and the fixed scratch size
+    before any device work: a norb * (nocc + nvirt) beyond the 64 MiB array and
+    invalid sizes return a status, not a launch."""
+    from dftbaby_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("HIP library not built")
+    lib = _lib.load()
+    fake = ctypes.c_void_p(1 << 20)             # never dereferenced
+    opa = (ctypes.c_int * 1)(4)
+    # norb = 3000 with the full space (nocc + nvirt = 3000): 72 MB of S C products
+    assert lib.dftb_trans_charges_noalloc(fake, fake, fake, opa, 750, 3000, 1500, 1500, fake, fake, fake,
+                                          None) == _lib.DFTB_ERR_WORKSPACE
+    assert lib.dftb_trans_charges_noalloc(fake, fake, fake, opa, 0, 4, 2, 2, fake, fake, fake, None) == \
+        _lib.DFTB_ERR_ARG
+    assert lib.dftb_trans_charges_noalloc(fake, fake, fake, opa, 1, 4, -1, 2, fake, fake, fake, None) == \
+        _lib.DFTB_ERR_ARG
