@@ -29,7 +29,7 @@ from .Solver import precision_code
 from . import _lib
 
 HARTREE_TO_EV = 27.211396132
-HARTREE_TO_NM = 45.5640
+HARTREE_TO_NM = 45.563352527    # lambda (nm) = HARTREE_TO_NM / (energy in Hartree), AtomicData.py:3
 EBOHR_TO_DEBYE = 2.54174623
 
 
@@ -202,9 +202,9 @@ class LR_TDDFTB:
         dev = op.device
         Om = torch.as_tensor(np.asarray(self.dftb2.getGaussianOverlap(), dtype=np.float64), device=dev)
         qii = op.diag_oo()
-        if qii is None:
-            qii = self._qii_from_charges()
         qaa = op.qaa
+        if qii is None or qaa is None:
+            raise RuntimeError("Lambda2 needs the diagonal charges q^ii_A and q^aa_A of the operator")
         o, v, nat = op.nocc, op.nvirt, op.nat
         X = torch.empty((o, nat), dtype=torch.float64, device=dev)
         gemm(qii, Om, X, o, nat, nat, nat, nat, nat)
@@ -224,10 +224,6 @@ class LR_TDDFTB:
         _lib.call("dftb_oia_normalize", _p(Oia), _p(Oii), _p(Oaa), o, v, op.opad, op.vpad, _stream())
         self.Oia_dev = Oia.reshape(-1)
         self.Oia = Oia[:o, :v].cpu().numpy()
-
-    def _qii_from_charges(self):
-        q_c = self.op.q["c"]
-        raise RuntimeError("occupied-occupied charges are needed for Lambda2")
 
     def _solve_lr_equations(self, response_method="Casida", multiplicity="S", nstates=None, diag_ifact=1,
                             diag_conv=1.0e-10, diag_maxiter=10, diag_check=1, diag_L2threshold=0.0,
@@ -293,7 +289,7 @@ class LR_TDDFTB:
         self._Cij_h = self._XmY_h = self._XpY_h = None
         self.Omega = w.cpu().numpy()
         if diag_check == 1 and nstates is not None:
-            Om_full, C_full, XmY_full, XpY_full = engine.casida_dense_gpu(op, omega_shift=omega_shift)
+            Om_full, C_full, XmY_full, XpY_full = engine.casida_dense_gpu(op, omega_shift=omega_shift, nev=nstates)
             Om_full = Om_full.cpu().numpy()
             err = np.sum(np.abs(self.Omega - Om_full[:nstates]))
             assert err < 1.0e-10, "eigenvalues differ, err = %s" % err
@@ -346,11 +342,17 @@ class LR_TDDFTB:
 
     # ----------------------------------------------------------- properties
     def _TransitionDipoles(self):
-        """d_I = sum_ia C^I_ia <i|r|a> (LR_TDDFTB.py:574-592).  The dipole matrix of
-        the homegrown parameter set is D^c = S o (R^c_m + R^c_n) / 2 (R: centre of
-        each AO), so <p|r_c|q> = (P_c^T Q + Q^T P_c) / 2 with Q = S C and
-        P_c = diag(R^c) C.  The active occupied x virtual block takes two GEMMs per
-        component; the full MO matrices tdipMO_X/Y/Z are formed on first access."""
+        """d_I = sum_ia C^I_ia <i|r|a> (LR_TDDFTB.py:574-592).
+
+        A ground state that exposes getOrbitalCenters (the bundled producer of
+        dftbaby_amd.groundstate) has the Mulliken point-charge dipole matrix
+        D^c = S o (R^c_m + R^c_n) / 2 (R: centre of each AO), so <p|r_c|q> =
+        (P_c^T Q + Q^T P_c) / 2 with Q = S C and P_c = diag(R^c) C: two GEMMs per
+        component for the active occupied x virtual block, the full MO matrices
+        tdipMO_X/Y/Z formed on first access.  Any other ground state (DFTBaby's
+        own, whose getDipoleMatrix comes from its Slater-Koster dipole tables,
+        DFTB2.py:2446-2452) goes through the dense C^T D C of
+        _TransitionDipolesDense."""
         op = self.op
         dev = op.device
         gs = self.dftb2
@@ -504,7 +506,9 @@ class LR_TDDFTB:
         return self.Lambda2
 
     def determineActiveSpace(self, threshold=1.0e-5):
-        """LR_TDDFTB.py:196-243 (host analysis of the k excitation vectors)."""
+        """LR_TDDFTB.py:196-243 (host analysis of the k excitation vectors): the
+        active space that holds 1 - threshold of every state's probability, with
+        the reference's report when the ground state is verbose."""
         Nst = self.Cij.shape[0]
         occ_mins, virt_maxs = [], []
         for n in range(Nst):
@@ -519,7 +523,28 @@ class LR_TDDFTB:
             occ_mins.append(self.active_occupied_orbs[occ.min()])
             virt_maxs.append(self.active_virtual_orbs[virt.max()])
         HOMO = self.occupied_orbs.max()
-        return HOMO - min(occ_mins) + 1, max(virt_maxs) - HOMO
+        nr_active_occ = HOMO - min(occ_mins) + 1
+        nr_active_virt = max(virt_maxs) - HOMO
+        if getattr(self.dftb2, "verbose", 0) > 0:
+            print(self.active_space_report(occ_mins, virt_maxs, nr_active_occ, nr_active_virt, threshold))
+        return nr_active_occ, nr_active_virt
+
+    def active_space_report(self, occ_mins, virt_maxs, nr_active_occ, nr_active_virt, threshold):
+        """The text determineActiveSpace prints (LR_TDDFTB.py:228-241)."""
+        Nst = len(occ_mins)
+        lines = ["", "Active Space", "============"]
+        for n in range(Nst):
+            lines.append("State %3d: contains excitations from window  occ %3d - virt %3d"
+                         % (n + 1, occ_mins[n], virt_maxs[n]))
+        lines += ["",
+                  "The suggested active space for describing the lowest %d states consists of" % Nst,
+                  "the highest %d occupied and the lowest %d virtual orbitals." % (nr_active_occ, nr_active_virt),
+                  "This active space accounts for %2.7f of the total probability for each state." % (1.0 - threshold),
+                  "You should check that the active space contains all neighbouring (almost) degenerate orbitals.",
+                  "Dimension of active space: %d" % (nr_active_occ * nr_active_virt),
+                  "Dimension of  full  space: %d" % (len(self.occupied_orbs) * len(self.virtual_orbs)),
+                  ""]
+        return "\n".join(lines)
 
     def _dominant_excitations(self, I, dominant_probability=0.99):
         C2 = np.abs(self.Cij[I]) ** 2
@@ -546,9 +571,10 @@ class LR_TDDFTB:
         txt = " ground state energy: %s %s\n\n" % (("%.7f hartree" % self.en0).rjust(15),
                                                    ("%.7f eV" % (self.en0 * HARTREE_TO_EV)).rjust(15))
         txt += "           Excited States\n           ==============\n"
-        txt += ("     Spn        exc. en. /hartree      exc. en. / eV   exc. en. / nm    osc. strength"
+        symstr = "   "          # no symmetry group: three blanks (LR_TDDFTB.py:768-771)
+        txt += ("     Spn %s     exc. en. /hartree      exc. en. / eV   exc. en. / nm    osc. strength"
                 "             dominant contrib.               en_KS / eV      transition dipole [TDx TDy TDz]"
-                "   /\\ diagn.\n")
+                "   /\\ diagn.\n" % symstr)
         orbe = self.dftb2.getKSEnergies()
         for I in range(min(print_max_states, len(self.Omega))):
             domex = self._dominant_excitations(I)
@@ -570,6 +596,26 @@ class LR_TDDFTB:
                 ("[%+2.7f %+2.7f %+2.7f]" % (self.tdipX[I], self.tdipY[I], self.tdipZ[I])).rjust(40),
                 ("%.4f" % self.Lambda2[I]).rjust(7))
         return txt
+
+    def saveTransitionCharges(self, save_transition_charges=None):
+        """Mulliken transition charges between the ground and each excited state,
+        q^I_A = sum_ia q^{ia}_A C^I_ia (one GEMM over the pair index), written as
+        LR_TDDFTB.py:1008-1026 does: three comment lines, then one row per atom and
+        one column per state, in units of -e (np.savetxt of -q)."""
+        if save_transition_charges is None:
+            return None
+        import os
+        op = self.op
+        k = self._C_dev.shape[0]
+        out = torch.empty((op.nat, k), dtype=torch.float64, device=op.device)
+        gram(op.q["c"].reshape(op.nat, op.N), op.nat, self._C_dev, k, op.N, out, op.ws)
+        tc = out.cpu().numpy()
+        with open(os.path.expandvars(os.path.expanduser(save_transition_charges)), "w") as fh:
+            fh.write("# Mulliken transition charges (in unit of -e) between ground and excited states\n")
+            fh.write("# ROWS: atoms\n")
+            fh.write("# COLUMNS: excited states\n")
+            np.savetxt(fh, -tc)
+        return tc
 
     def saveAbsorptionSpectrum(self, spectrum_file="absorption_spectrum.dat"):
         """LR_TDDFTB.py:807-832 (no symmetry labels: one file)."""

@@ -940,10 +940,11 @@ def hermitian_davidson_gpu(op, nstates, omega_shift=None, ifact=1, conv=1.0e-14,
     return w, C, XmY, XpY, stats
 
 
-def casida_dense_gpu(op, chunk=512, omega_shift=None):
-    """Full Casida solve (Solver.py:98-181) for small systems: all n_ov roots.
-    omega_shift (o x v, the CT correction) is added to the diagonals of A and B
-    (Solver.py:46-49), i.e. twice to A+B.
+def casida_dense_gpu(op, chunk=512, omega_shift=None, nev=None):
+    """Full Casida solve (Solver.py:98-181) for small systems: all n_ov roots, or
+    the lowest nev of them (the diag_check comparison, LR_TDDFTB.py:493-518, needs
+    only nstates).  omega_shift (o x v, the CT correction) is added to the
+    diagonals of A and B (Solver.py:46-49), i.e. twice to A+B.
 
     (A+-B) are assembled column block by column block with the matvec kernel,
     (A-B)^{1/2} is diagonal (lc = 0) or the Newton-Schulz square root, and the
@@ -988,30 +989,31 @@ def casida_dense_gpu(op, chunk=512, omega_shift=None):
         X = torch.empty_like(ApB)
         gemm(ApB, Sq, X, n, n, n, n, n, n)
         gemm(Sq, X, R, n, n, n, n, n, n)
-    w2 = torch.empty(n, dtype=torch.float64, device=dev)
-    F = torch.empty((n, n), dtype=torch.float64, device=dev)
-    w, wn = ws.get(_lib.load().dftb_syevx_workspace(n, n))
-    call("dftb_syevx", _p(R), n, n, n, _p(w2), _p(F), _p(w), wn, st)
+    m = n if nev is None else max(1, min(int(nev), n))
+    w2 = torch.empty(m, dtype=torch.float64, device=dev)
+    F = torch.empty((n, m), dtype=torch.float64, device=dev)
+    w, wn = ws.get(_lib.load().dftb_syevx_workspace(n, m))
+    call("dftb_syevx", _p(R), n, n, m, _p(w2), _p(F), _p(w), wn, st)
     Om = torch.sqrt(w2)
     # X+Y = (A-B)^{1/2} F / sqrt(Omega); X-Y = (A+B)(X+Y)/Omega; C = (A-B)^{-1/2} (X+Y) sqrt(Omega)
     XpYc = torch.empty_like(F)
     if Sq is None:
-        call("dftb_scale2", _p(F), _p(sq), _p(1.0 / torch.sqrt(Om)), n, n, _p(XpYc), st)
+        call("dftb_scale2", _p(F), _p(sq), _p(1.0 / torch.sqrt(Om)), n, m, _p(XpYc), st)
     else:
-        gemm(Sq, F, XpYc, n, n, n, n, n, n)
-        call("dftb_scale2", _p(XpYc), None, _p(1.0 / torch.sqrt(Om)), n, n, _p(XpYc), st)
+        gemm(Sq, F, XpYc, n, m, n, n, m, m)
+        call("dftb_scale2", _p(XpYc), None, _p(1.0 / torch.sqrt(Om)), n, m, _p(XpYc), st)
     XmYc = torch.empty_like(F)
-    gemm(ApB, XpYc, XmYc, n, n, n, n, n, n)
-    call("dftb_scale2", _p(XmYc), None, _p(1.0 / Om), n, n, _p(XmYc), st)
+    gemm(ApB, XpYc, XmYc, n, m, n, n, m, m)
+    call("dftb_scale2", _p(XmYc), None, _p(1.0 / Om), n, m, _p(XmYc), st)
     Cc = torch.empty_like(F)
     if Sq is None:
-        call("dftb_scale2", _p(F), None, None, n, n, _p(Cc), st)      # C = F when A-B is diagonal
+        call("dftb_scale2", _p(F), None, None, n, m, _p(Cc), st)      # C = F when A-B is diagonal
     else:
-        gemm(Sinv, XpYc, Cc, n, n, n, n, n, n)
-        call("dftb_scale2", _p(Cc), None, _p(torch.sqrt(Om)), n, n, _p(Cc), st)
+        gemm(Sinv, XpYc, Cc, n, m, n, n, m, m)
+        call("dftb_scale2", _p(Cc), None, _p(torch.sqrt(Om)), n, m, _p(Cc), st)
 
     def padded(Xc):   # columns are states -> rows [state][opad*vpad]
-        out = torch.zeros((n, N), dtype=torch.float64, device=dev)
+        out = torch.zeros((m, N), dtype=torch.float64, device=dev)
         out[:, real] = Xc.T
         return out
     return Om, padded(Cc), padded(XmYc), padded(XpYc)

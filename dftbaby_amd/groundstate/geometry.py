@@ -172,3 +172,18 @@ def benzene(cc=1.39, ch=1.09):
         t = np.deg2rad(60.0 * k)
         atoms.append((1, [(cc + ch) * np.cos(t), (cc + ch) * np.sin(t), 0.0]))
     return [(Z, list(np.array(x) * ANGSTROM)) for Z, x in atoms]
+
+
+def polyene_ensemble(count=256, seed=20261202, sigma=0.05):
+    """Uneven ensemble of all-trans polyenes C_nH_{n+2}: n drawn uniformly from
+    the even values 8..40, each geometry with its own seeded Gaussian
+    displacement (sigma in Angstrom).  Returns [(n, atoms)]."""
+    rng = np.random.default_rng(seed)
+    ns = rng.choice(np.arange(8, 41, 2), size=count)
+    return [(int(n), displace(polyene(int(n)), sigma, seed + 1 + i)) for i, n in enumerate(ns)]
+
+
+def flake_c604h68(sigma=0.02, seed=20261202):
+    """The armchair graphene flake C604H68 (widths 18.5, 22.5, 22.5 A; Nat 672,
+    Norb 2484) with seeded Gaussian displacements of sigma Angstrom."""
+    return displace(graphene_flake_armchair((18.5, 22.5, 22.5)), sigma, seed)

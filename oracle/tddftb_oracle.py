@@ -57,6 +57,29 @@ def active_orbitals(f, orbe, nr_active_occ=None, nr_active_virt=None):
     return occupied, virtual, act_occ, act_virt
 
 
+def active_lm_orbitals(orbs, f, orbe, atomlist, valorbs, nr_active_occ=1000, nr_active_virt=1000,
+                       select_lm=((1, 0),)):
+    """setActiveLMOrbitals (LR_TDDFTB.py:125-166): only MOs with weight
+    sum |C_mu,p|^2 > 1e-15 on AOs whose (l, m) is in select_lm enter the
+    occupied/virtual sets; then the same top-k selection as active_orbitals."""
+    lm = []
+    iao = 0
+    for Zi, _ in atomlist:
+        for (ni, li, mi) in valorbs[Zi]:
+            if (li, mi) in list(select_lm):
+                lm.append(iao)
+            iao += 1
+    lm = np.array(lm, dtype=int)
+    is_lm = np.zeros(orbs.shape[1], dtype=bool)
+    for imo in range(orbs.shape[1]):
+        is_lm[imo] = np.sum(np.abs(orbs[lm, imo]) ** 2) > 1.0e-15
+    occupied = np.where((f > 0.1) & is_lm)[0]
+    virtual = np.where((f <= 0.1) & is_lm)[0]
+    act_occ = occupied[np.argsort(orbe[occupied], kind="stable")[::-1][:nr_active_occ][::-1]]
+    act_virt = virtual[np.argsort(orbe[virtual], kind="stable")[:nr_active_virt]]
+    return occupied, virtual, act_occ, act_virt
+
+
 # ---------------------------------------------------------------------------
 # a2: Mulliken transition charges (tddftb.f90:12-112, LR_TDDFTB.py:257-319)
 # ---------------------------------------------------------------------------
@@ -521,11 +544,14 @@ def Lambda2(Cij, Oia):
 
 
 def transition_dipoles(Cij, orbs, DipMat, act_occ, act_virt):
-    """d_I = sum_ia C^I_ia <i|r|a>  (LR_TDDFTB.py:574-592)."""
+    """d_I = sum_ia C^I_ia <i|r|a>  (LR_TDDFTB.py:574-592).  The reference forms
+    the full MO matrices orbs^T D orbs and selects the active block; the block
+    orbs[:, occ]^T D orbs[:, virt] is formed directly (same numbers)."""
     out = []
+    Co, Cv = orbs[:, act_occ], orbs[:, act_virt]
     for c in range(3):
-        tMO = orbs.T @ DipMat[:, :, c] @ orbs
-        tact = tMO[np.ix_(act_occ, act_virt)]
+        Dc = np.ascontiguousarray(DipMat[:, :, c])
+        tact = Co.T @ (Dc @ Cv)
         out.append(np.tensordot(Cij, tact, axes=([1, 2], [0, 1])))
     return tuple(out)
 
@@ -551,13 +577,17 @@ def oscillator_strengths(Omega, tdip, multiplicity="S"):
 # driver mirroring LR_TDDFTB._solve_lr_equations (LR_TDDFTB.py:403-527)
 # ---------------------------------------------------------------------------
 def solve_from_ground_state(gs, nstates=None, nr_active_occ=None, nr_active_virt=None,
-                            diag_ifact=1, diag_conv=1.0e-5, diag_maxiter=50, oszis="Dipoles"):
+                            diag_ifact=1, diag_conv=1.0e-5, diag_maxiter=50, oszis="Dipoles", select_lm=None):
     """Full oracle solve on a ground-state object exposing the DFTB2 getters."""
     f = gs.getOccupation()
     orbe = gs.getKSEnergies()
     orbs = gs.getKSCoefficients()
     S = gs.getOverlapMatrix()
-    occ, virt, ao, av = active_orbitals(f, orbe, nr_active_occ, nr_active_virt)
+    if select_lm is not None:
+        occ, virt, ao, av = active_lm_orbitals(orbs, f, orbe, gs.getGeometry(), gs.getValorbs(), nr_active_occ,
+                                               nr_active_virt, select_lm)
+    else:
+        occ, virt, ao, av = active_orbitals(f, orbe, nr_active_occ, nr_active_virt)
     q_oo, q_ov, q_vv = trans_charges(orbs[:, ao], orbs[:, av], S, gs.orbsPerAtom)
     Oia = Lambda2_calcOia(q_oo, q_vv, gs.getGaussianOverlap())
     gamma = gs.getGamma()

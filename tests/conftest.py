@@ -51,6 +51,13 @@ def ground_state(name, lc, sigma=0.0, seed=0):
     return _GS_CACHE[key]
 
 
+def progress(msg):
+    """A progress line on the process's stderr, past pytest's output capture, so a
+    long GPU test shows it is working."""
+    sys.__stderr__.write("[test] %s\n" % msg)
+    sys.__stderr__.flush()
+
+
 @pytest.fixture
 def rng():
     return np.random.default_rng(1234)

@@ -436,3 +436,33 @@ def test_active_orbitals_truncation_bruteforce(seed):
     occ = np.where(f > 0.1)[0]
     wrong = occ[np.argsort(orbe[occ], kind="stable")[:3]]
     assert not np.array_equal(wrong, _active_bruteforce(f, orbe, 3, 4)[0])
+
+
+def test_active_lm_orbitals_pi_system_of_planar_polyene():
+    """setActiveLMOrbitals (LR_TDDFTB.py:125-166) with select_lm = [(1, 0)] (p_z)
+    on the planar (z = 0) C8H10: the pi system is exactly the 8 MOs with p_z
+    weight (4 occupied, 4 virtual); the oracle and the drop-in's host selection
+    agree with a count of p_z weights, and truncation keeps the frontier pi
+    orbitals."""
+    from dftbaby_amd.LR_TDDFTB import LR_TDDFTB
+    gs = ground_state("C8H10", 1)
+    orbs, f, orbe = gs.getKSCoefficients(), gs.getOccupation(), gs.getKSEnergies()
+    pz = [mu for mu, (A, k) in enumerate((A, k) for A, (Z, _) in enumerate(gs.getGeometry())
+                                         for k in range(len(gs.getValorbs()[Z])))
+          if gs.getValorbs()[gs.getGeometry()[A][0]][k][1:] == (1, 0)]
+    assert len(pz) == 8
+    w = np.array([sum(orbs[mu, p] ** 2 for mu in pz) for p in range(orbs.shape[1])])
+    pi = np.where(w > 1e-15)[0]
+    assert len(pi) == 8 and np.all((w < 1e-20) | (w > 1e-3))
+    occ, virt, ao, av = O.active_lm_orbitals(orbs, f, orbe, gs.getGeometry(), gs.getValorbs(), 1000, 1000, [(1, 0)])
+    np.testing.assert_array_equal(occ, pi[f[pi] > 0.1])
+    np.testing.assert_array_equal(virt, pi[f[pi] <= 0.1])
+    assert len(ao) == 4 and len(av) == 4
+    _, _, ao2, av2 = O.active_lm_orbitals(orbs, f, orbe, gs.getGeometry(), gs.getValorbs(), 2, 3, [(1, 0)])
+    np.testing.assert_array_equal(ao2, ao[-2:])
+    np.testing.assert_array_equal(av2, av[:3])
+    td = LR_TDDFTB(ground_state=gs)
+    td.setActiveLMOrbitals(2, 3, select_lm=[(1, 0)])
+    np.testing.assert_array_equal(td.active_occupied_orbs, ao2)
+    np.testing.assert_array_equal(td.active_virtual_orbs, av2)
+    np.testing.assert_array_equal(td.occupied_orbs, occ)
