@@ -2,55 +2,100 @@
 //
 // q^{pq}_A = 1/2 sum_{mu in A} [X_{mu p} Y_{mu q} + X_{mu q} Y_{mu p}],  X = C, Y = S C
 // (DFTB/extensions/tddftb.f90:50-110).  S C is one MFMA GEMM; the charge kernel
-// is bound by its HBM writes (Nat * P * Q outputs, each 2*nmu FMAs).  One
-// workgroup owns one atom A and a block of PB p-orbitals: the atom's rows of
-// X and Y for those p are staged in LDS once, then every thread streams over
-// q reading X_{mu q}, Y_{mu q} (coalesced, reused PB times) and writes the PB
-// outputs q^{p q}_A for its q (coalesced along q).
+// is bound by its HBM writes (Nat * P * Q outputs, each 2*nmu FMAs); its reads
+// (X, Y rows of one atom per workgroup) are small next to them.
 #include <mutex>
 #include <vector>
 #include "common.h"
 #include "gemm.h"
 
-#define TC_PB 16
 #define TC_MAXMU 16
+#define TC_LDS_BYTES (48 * 1024)
 
-template <int NMU>
+// One workgroup owns one atom A, a chunk of p and S stripes of 256 q (one q per
+// thread and stripe).  The atom's p-rows X_{mu p}, Y_{mu p} of the chunk are
+// staged in LDS once, as (x, y) pairs per p with zeros for mu >= nmu; each
+// thread keeps X_{mu q}, Y_{mu q} of its S q in registers (read once).  The
+// loop runs over p and, inside, over the stripes, so a workgroup completes
+// whole output rows (S * 256 >= the row length, or S stripes of it) at a time:
+// partial 128-byte lines at stripe boundaries are merged in L2 before they
+// reach HBM, which matters when rows are not line-aligned (the reference layout
+// with an odd-sized orbital block).  LDS reads are wave-uniform broadcasts and
+// the loop has no barrier.  Rows p in [P, Ppad) and columns q in [Q, Qpad) are
+// written as zeros (the engine layouts' padding).  Dynamic LDS: 2 * NMU * pchunk
+// doubles.
+template <int NMU, int S>
 __global__ __launch_bounds__(256) void tc_kernel(const double* __restrict__ Xp, const double* __restrict__ Yp, int P,
                                                  const double* __restrict__ Xq, const double* __restrict__ Yq, int Q,
-                                                 const int* __restrict__ atom_off, int Ppad, int Qpad,
+                                                 const int* __restrict__ atom_off, int Ppad, int Qpad, int pchunk,
                                                  double* __restrict__ out, long sP, long sA) {
+  extern __shared__ double sxy[];        // [pchunk][NMU][2]
   const int A = blockIdx.x;
-  const int p0 = blockIdx.y * TC_PB;
+  const int pbeg = blockIdx.y * pchunk;
+  const int pend = min(Ppad, pbeg + pchunk);
+  const int qb = blockIdx.z * S * 256 + threadIdx.x;
   const int mu0 = atom_off[A];
   const int nmu = atom_off[A + 1] - mu0;
-  __shared__ double sx[NMU][TC_PB];
-  __shared__ double sy[NMU][TC_PB];
-  for (int t = threadIdx.x; t < NMU * TC_PB; t += blockDim.x) {
-    int mu = t / TC_PB, pp = t % TC_PB, p = p0 + pp;
-    bool ok = (mu < nmu) && (p < P);
-    sx[mu][pp] = ok ? Xp[(long)(mu0 + mu) * P + p] : 0.0;
-    sy[mu][pp] = ok ? Yp[(long)(mu0 + mu) * P + p] : 0.0;
+  // staging: an index outside the atom's rows (or beyond P) is clamped to
+  // element 0 and its value masked, so the loads carry no branch
+  for (int t = threadIdx.x; t < (pend - pbeg) * NMU; t += blockDim.x) {
+    const int pp = t / NMU, mu = t % NMU, p = pbeg + pp;
+    const bool ok = (mu < nmu) && (p < P);
+    const long idx = ok ? (long)(mu0 + mu) * P + p : 0;
+    const double x = Xp[idx], y = Yp[idx];
+    sxy[2 * t] = ok ? x : 0.0;
+    sxy[2 * t + 1] = ok ? y : 0.0;
   }
   __syncthreads();
-  const int pend = min(Ppad, p0 + TC_PB);
-  for (int q = threadIdx.x; q < Qpad; q += blockDim.x) {
-    double xq[NMU], yq[NMU];
-    const bool inq = q < Q;
+  double xq[S][NMU], yq[S][NMU];
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    const int q = qb + 256 * j;
 #pragma unroll
     for (int mu = 0; mu < NMU; ++mu) {
-      bool ok = inq && (mu < nmu);
-      xq[mu] = ok ? Xq[(long)(mu0 + mu) * Q + q] : 0.0;
-      yq[mu] = ok ? Yq[(long)(mu0 + mu) * Q + q] : 0.0;
-    }
-    for (int p = p0; p < pend; ++p) {
-      const int pp = p - p0;
-      double s = 0.0;
-#pragma unroll
-      for (int mu = 0; mu < NMU; ++mu) s += sx[mu][pp] * yq[mu] + xq[mu] * sy[mu][pp];
-      out[(long)p * sP + (long)A * sA + q] = (p < P && inq) ? 0.5 * s : 0.0;
+      const bool ok = (q < Q) && (mu < nmu);
+      const long idx = ok ? (long)(mu0 + mu) * Q + q : 0;
+      const double x = Xq[idx], y = Yq[idx];
+      xq[j][mu] = ok ? x : 0.0;
+      yq[j][mu] = ok ? y : 0.0;
     }
   }
+  double* o = out + (long)A * sA + (long)pbeg * sP + qb;
+  const double* r = sxy;
+  for (int p = pbeg; p < pend; ++p, r += 2 * NMU, o += sP) {
+    double rv[2 * NMU];
+#pragma unroll
+    for (int i = 0; i < 2 * NMU; ++i) rv[i] = r[i];
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int mu = 0; mu < NMU; ++mu) s += rv[2 * mu] * yq[j][mu] + xq[j][mu] * rv[2 * mu + 1];
+      if (qb + 256 * j < Qpad) o[256 * j] = 0.5 * s;   // zero for padded p (staged zeros) and q (zero registers)
+    }
+  }
+}
+
+template <int NMU, int S>
+static void tc_launch(dim3 grid, size_t lds, hipStream_t st, const double* Xp, const double* Yp, int P,
+                      const double* Xq, const double* Yq, int Q, const int* off, int Ppad, int Qpad, int pchunk,
+                      double* out, long sP, long sA) {
+  hipLaunchKernelGGL((tc_kernel<NMU, S>), grid, dim3(256), lds, st, Xp, Yp, P, Xq, Yq, Q, off, Ppad, Qpad, pchunk,
+                     out, sP, sA);
+}
+
+// stripes per workgroup held in registers: up to 8 (NMU 4), 4 (NMU 9), 2 (NMU 16)
+template <int NMU>
+static void tc_dispatch(int S, dim3 grid, size_t lds, hipStream_t st, const double* Xp, const double* Yp, int P,
+                        const double* Xq, const double* Yq, int Q, const int* off, int Ppad, int Qpad, int pchunk,
+                        double* out, long sP, long sA) {
+#define TC_CASE(n) \
+  case n: tc_launch<NMU, (NMU * n <= 32 ? n : 1)>(grid, lds, st, Xp, Yp, P, Xq, Yq, Q, off, Ppad, Qpad, pchunk, out, sP, sA); break;
+  switch (S) {
+    TC_CASE(1) TC_CASE(2) TC_CASE(3) TC_CASE(4) TC_CASE(5) TC_CASE(6) TC_CASE(7) TC_CASE(8)
+    default: break;
+  }
+#undef TC_CASE
 }
 
 struct TcPlan {
@@ -70,13 +115,35 @@ static int tc_block(const TcPlan& t, bool p_occ, bool q_occ, int Ppad, int Qpad,
   int P = p_occ ? t.nocc : t.nvirt;
   int Q = q_occ ? t.nocc : t.nvirt;
   if (P == 0 || Ppad == 0 || Qpad == 0) return DFTB_OK;
-  dim3 grid(t.nat, ceil_div(Ppad, TC_PB));
-  if (t.maxmu <= 4)
-    hipLaunchKernelGGL(tc_kernel<4>, grid, dim3(256), 0, st, Xp, Yp, P, Xq, Yq, Q, t.off, Ppad, Qpad, out, sP, sA);
-  else if (t.maxmu <= 9)
-    hipLaunchKernelGGL(tc_kernel<9>, grid, dim3(256), 0, st, Xp, Yp, P, Xq, Yq, Q, t.off, Ppad, Qpad, out, sP, sA);
+  const int nmu = t.maxmu <= 4 ? 4 : (t.maxmu <= 9 ? 9 : 16);
+  const int smax = 32 / nmu;                       // stripes of 256 q per workgroup (registers)
+  const int stripes = ceil_div(Qpad, 256);
+  const int S = stripes < smax ? stripes : smax;
+  const int gz = ceil_div(stripes, S);
+  // p chunks: output rows that are not 128-byte aligned (the reference layout
+  // with nocc or nvirt not a multiple of 16) leave partial lines at row ends,
+  // merged best by short chunks (16 rows: 4.8 TB/s against 3.5 TB/s with 624 at
+  // the flake); aligned rows stream best in long chunks (>= ~1024 workgroups)
+  const int pmax = (int)(TC_LDS_BYTES / (2 * nmu * sizeof(double)));
+  int pchunk;
+  if ((sP % 16) != 0 || (sA % 16) != 0) {
+    pchunk = 16;
+  } else {
+    const long wg = (long)t.nat * gz;
+    int psplit = (int)((1024 + wg - 1) / wg);
+    if (psplit < 1) psplit = 1;
+    pchunk = ceil_div(Ppad, psplit);
+  }
+  if (pchunk > pmax) pchunk = pmax;
+  if (pchunk < 1) pchunk = 1;
+  const size_t lds = (size_t)2 * nmu * pchunk * sizeof(double);
+  dim3 grid(t.nat, ceil_div(Ppad, pchunk), gz);
+  if (nmu == 4)
+    tc_dispatch<4>(S, grid, lds, st, Xp, Yp, P, Xq, Yq, Q, t.off, Ppad, Qpad, pchunk, out, sP, sA);
+  else if (nmu == 9)
+    tc_dispatch<9>(S, grid, lds, st, Xp, Yp, P, Xq, Yq, Q, t.off, Ppad, Qpad, pchunk, out, sP, sA);
   else
-    hipLaunchKernelGGL(tc_kernel<16>, grid, dim3(256), 0, st, Xp, Yp, P, Xq, Yq, Q, t.off, Ppad, Qpad, out, sP, sA);
+    tc_dispatch<16>(S, grid, lds, st, Xp, Yp, P, Xq, Yq, Q, t.off, Ppad, Qpad, pchunk, out, sP, sA);
   DFTB_CHECK_LAUNCH();
   return DFTB_OK;
 }

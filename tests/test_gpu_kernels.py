@@ -384,6 +384,45 @@ def test_trans_charges_reference_layout(name, lc):
         np.testing.assert_allclose(got.cpu().numpy(), ref, rtol=0, atol=1e-13)
 
 
+def test_trans_charges_flake_invariants():
+    """Reference-layout charges of the flake C604H68 (Nat 672, o = v = 1242: 25 GB
+    of outputs, prefilled with NaN): sum_A q^{pq}_A = (C^T S C)_{pq} = delta_pq for
+    every (p, q) of the three blocks, q^{pq} = q^{qp}, and eight atoms (first,
+    last, six random) against the closed form evaluated on the host."""
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _p, _stream, Workspace
+    from dftbaby_amd.groundstate import geometry as g
+    from dftbaby_amd.groundstate.dftb2 import DFTB2
+    gs = DFTB2(g.flake_c604h68(), long_range_correction=1, long_range_radius=3.03)
+    gs.getEnergy(scf_conv=1e-8)
+    f, orbe, C, S = gs.getOccupation(), gs.getKSEnergies(), gs.getKSCoefficients(), gs.getOverlapMatrix()
+    _, _, ao, av = O.active_orbitals(f, orbe)
+    nat, norb, o, v = len(gs.orbsPerAtom), S.shape[0], len(ao), len(av)
+    Co, Cv, St = T(C[:, ao]), T(C[:, av]), T(S)
+    q = [torch.full(sz, float("nan"), dtype=torch.float64, device=_dev()) for sz in ((nat, o, o), (nat, o, v),
+                                                                                    (nat, v, v))]
+    opa = np.ascontiguousarray(gs.orbsPerAtom, dtype=np.int32)
+    ws = Workspace(_dev())
+    w, wn = ws.get(_lib.load().dftb_trans_charges_workspace(norb, o, v, nat))
+    _lib.call("dftb_trans_charges", _p(Co), _p(Cv), _p(St), opa.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+              nat, norb, o, v, _p(q[0]), _p(q[1]), _p(q[2]), _p(w), wn, _stream())
+    eye_o = torch.eye(o, dtype=torch.float64, device=_dev())
+    eye_v = torch.eye(v, dtype=torch.float64, device=_dev())
+    assert float((q[0].sum(0) - eye_o).abs().max()) < 1e-12
+    assert float(q[1].sum(0).abs().max()) < 1e-12
+    assert float((q[2].sum(0) - eye_v).abs().max()) < 1e-12
+    assert float((q[0] - q[0].transpose(1, 2)).abs().max()) < 1e-15      # rounding of the FMA order only
+    assert float((q[2] - q[2].transpose(1, 2)).abs().max()) < 1e-15
+    off = np.concatenate([[0], np.cumsum(opa)])
+    SC = S @ C
+    rng = np.random.default_rng(3)
+    for A in [0, nat - 1] + list(rng.integers(1, nat - 1, 6)):
+        sl = slice(off[A], off[A + 1])
+        for (blk, P_, Q_) in ((0, ao, ao), (1, ao, av), (2, av, av)):
+            ref = 0.5 * (C[sl][:, P_].T @ SC[sl][:, Q_] + SC[sl][:, P_].T @ C[sl][:, Q_])
+            np.testing.assert_allclose(q[blk][A].cpu().numpy(), ref, rtol=0, atol=1e-13)
+
+
 def test_trans_charges_engine_layout():
     from dftbaby_amd.engine import CasidaOperatorGPU
     gs = ground_state("C8H10", 1, 0.05, 7)
