@@ -83,9 +83,12 @@ _SIGS = {
     "dftb_convert_f32": (_i, [_vp, _l, _vp, _vp]),
     "dftb_convert_bf16pair": (_i, [_vp, _l, _vp, _vp, _vp]),
     "dftb_convert_bf16pair_atom_major": (_i, [_vp, _l, _l, _i, _l, _vp, _vp, _vp]),
-    "dftb_residual": (_i, [_vp, _vp, _vp, _i, _l, _vp, _vp, _vp]),
+    "dftb_norms_workspace": (_sz, [_i]),
+    "dftb_residual": (_i, [_vp, _vp, _vp, _i, _l, _vp, _vp, _vp, _sz, _vp]),
+    "dftb_gram_workspace": (_sz, [_i, _i, _l]),
+    "dftb_gram": (_i, [_vp, _l, _i, _vp, _l, _i, _l, _vp, _l, _vp, _sz, _vp]),
     "dftb_precondition": (_i, [_vp, _vp, _vp, _i, _vp, _l, _vp, _vp]),
-    "dftb_row_norms": (_i, [_vp, _i, _l, _vp, _vp]),
+    "dftb_row_norms": (_i, [_vp, _i, _l, _vp, _vp, _sz, _vp]),
     "dftb_scale_rows": (_i, [_vp, _vp, _i, _l, _vp]),
     "dftb_scale2": (_i, [_vp, _vp, _vp, _i, _l, _vp, _vp]),
     "dftb_weighted_sumsq": (_i, [_vp, _vp, _i, _l, _vp, _vp]),

@@ -31,34 +31,57 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return s;
 }
 
-// W[j] = P[j] - w[j] X[j]; norms[j] = ||W[j]||
-__global__ __launch_bounds__(1024) void residual_kernel(const double* __restrict__ P, const double* __restrict__ X,
-                                                        const double* __restrict__ w, long n, double* __restrict__ W,
-                                                        double* __restrict__ norms) {
-  __shared__ double sh[16];
-  const long j = blockIdx.x;
+// Residuals and row norms stream k vectors of n doubles.  Each vector is split
+// over NORM_SLICES workgroups (enough to keep every CU streaming for the k = 20
+// to 40 vectors of a Davidson step); each slice writes its partial sum of
+// squares, and a second kernel adds the slices of each vector in a fixed order
+// (deterministic).  The caller's workspace holds k * NORM_SLICES partials.
+#define NORM_SLICES 32
+
+// W[j] = P[j] - w[j] X[j]; partial[j][s] = sum over slice s of W[j]^2
+__global__ __launch_bounds__(256) void residual_kernel(const double* __restrict__ P, const double* __restrict__ X,
+                                                       const double* __restrict__ w, long n, double* __restrict__ W,
+                                                       double* __restrict__ partial) {
+  __shared__ double sh[4];
+  const long j = blockIdx.y;
+  const long chunk = (n + gridDim.x - 1) / gridDim.x;
+  const long b = blockIdx.x * chunk;
+  const long e = b + chunk < n ? b + chunk : n;
   const double wj = w[j];
   const double* p = P + j * n;
   const double* x = X + j * n;
   double* o = W + j * n;
   double s = 0.0;
-  for (long i = threadIdx.x; i < n; i += blockDim.x) {
-    double r = p[i] - wj * x[i];
+  for (long i = b + threadIdx.x; i < e; i += blockDim.x) {
+    const double r = p[i] - wj * x[i];
     o[i] = r;
     s += r * r;
   }
   s = block_sum(s, sh);
-  if (threadIdx.x == 0) norms[j] = sqrt(s);
+  if (threadIdx.x == 0) partial[j * gridDim.x + blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(1024) void row_norms_kernel(const double* __restrict__ X, long n, double* __restrict__ norms) {
-  __shared__ double sh[16];
-  const long j = blockIdx.x;
+__global__ __launch_bounds__(256) void row_norms_kernel(const double* __restrict__ X, long n,
+                                                        double* __restrict__ partial) {
+  __shared__ double sh[4];
+  const long j = blockIdx.y;
+  const long chunk = (n + gridDim.x - 1) / gridDim.x;
+  const long b = blockIdx.x * chunk;
+  const long e = b + chunk < n ? b + chunk : n;
   const double* x = X + j * n;
   double s = 0.0;
-  for (long i = threadIdx.x; i < n; i += blockDim.x) s += x[i] * x[i];
+  for (long i = b + threadIdx.x; i < e; i += blockDim.x) s += x[i] * x[i];
   s = block_sum(s, sh);
-  if (threadIdx.x == 0) norms[j] = sqrt(s);
+  if (threadIdx.x == 0) partial[j * gridDim.x + blockIdx.x] = s;
+}
+
+// norms[j] = sqrt(sum_s partial[j][s]), slices added in order
+__global__ void slice_norms_kernel(const double* __restrict__ partial, int k, int slices, double* __restrict__ norms) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  double s = 0.0;
+  for (int t = 0; t < slices; ++t) s += partial[(long)j * slices + t];
+  norms[j] = sqrt(s);
 }
 
 __global__ void precondition_kernel(const double* __restrict__ W, const double* __restrict__ w,
@@ -89,19 +112,32 @@ __global__ void unit_vectors_kernel(const long* idx, int k, long n, double* X) {
   }
 }
 
+extern "C" size_t dftb_norms_workspace(int k) { return (size_t)(k > 0 ? k : 0) * NORM_SLICES * sizeof(double); }
+
 extern "C" int dftb_residual(const double* P, const double* X, const double* w, int k, long n, double* W, double* norms,
-                             void* stream) {
+                             void* workspace, size_t ws_bytes, void* stream) {
   if (k <= 0) return DFTB_OK;
-  if (!P || !X || !w || !W || !norms) return DFTB_ERR_ARG;
-  hipLaunchKernelGGL(residual_kernel, dim3(k), dim3(1024), 0, (hipStream_t)stream, P, X, w, n, W, norms);
+  if (!P || !X || !w || !W || !norms || !workspace || n <= 0) return DFTB_ERR_ARG;
+  if (ws_bytes < dftb_norms_workspace(k)) return DFTB_ERR_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  double* part = (double*)workspace;
+  hipLaunchKernelGGL(residual_kernel, dim3(NORM_SLICES, k), dim3(256), 0, st, P, X, w, n, W, part);
+  hipLaunchKernelGGL(slice_norms_kernel, dim3((k + 63) / 64), dim3(64), 0, st, (const double*)part, k, NORM_SLICES,
+                     norms);
   DFTB_CHECK_LAUNCH();
   return DFTB_OK;
 }
 
-extern "C" int dftb_row_norms(const double* X, int k, long n, double* norms, void* stream) {
+extern "C" int dftb_row_norms(const double* X, int k, long n, double* norms, void* workspace, size_t ws_bytes,
+                              void* stream) {
   if (k <= 0) return DFTB_OK;
-  if (!X || !norms) return DFTB_ERR_ARG;
-  hipLaunchKernelGGL(row_norms_kernel, dim3(k), dim3(1024), 0, (hipStream_t)stream, X, n, norms);
+  if (!X || !norms || !workspace || n <= 0) return DFTB_ERR_ARG;
+  if (ws_bytes < dftb_norms_workspace(k)) return DFTB_ERR_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  double* part = (double*)workspace;
+  hipLaunchKernelGGL(row_norms_kernel, dim3(NORM_SLICES, k), dim3(256), 0, st, X, n, part);
+  hipLaunchKernelGGL(slice_norms_kernel, dim3((k + 63) / 64), dim3(64), 0, st, (const double*)part, k, NORM_SLICES,
+                     norms);
   DFTB_CHECK_LAUNCH();
   return DFTB_OK;
 }
@@ -240,6 +276,65 @@ static int gram(const double* X, int P, const double* Y, int Q, long n, const Or
   int rc = gemm_launch(g, st);
   if (rc) return rc;
   return gemm_reduce_launch((double*)(ws + o.part), (long)P * Q, o.ks, out, (long)P * Q, 1.0, 0.0, st);
+}
+
+// out[p * ldo + q] = sum_s part[s][p][q]  (slices added in order)
+__global__ void gram_reduce_kernel(const double* __restrict__ part, int ks, int P, int Q, double* __restrict__ out,
+                                   long ldo) {
+  const long PQ = (long)P * Q;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < PQ; i += (long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int t = 0; t < ks; ++t) s += part[t * PQ + i];
+    const long p = i / Q, q = i - p * Q;
+    out[p * ldo + q] = s;
+  }
+}
+
+static OrthoLayout gram_layout(int P, int Q, long n) {
+  // enough K slices to fill the GPU (>= ~2048 workgroups of 64 x 64 tiles) with
+  // chunks of >= 2048 elements, so the small operand's chunk stays in L2 across
+  // the row tiles
+  OrthoLayout o = {};
+  const long tiles = (long)((P + 63) / 64) * ((Q + 63) / 64);
+  long ks = (n + 2047) / 2048;
+  const long want = (2048 + tiles - 1) / tiles;
+  if (ks > want) ks = want;
+  if (ks > 512) ks = 512;
+  if (ks < 1) ks = 1;
+  o.kchunk = ((n + ks - 1) / ks + 31) / 32 * 32;
+  o.ks = (int)((n + o.kchunk - 1) / o.kchunk);
+  o.part = 0;
+  o.total = al256((size_t)o.ks * P * Q * 8);
+  return o;
+}
+
+extern "C" size_t dftb_gram_workspace(int P, int Q, long n) {
+  if (P <= 0 || Q <= 0 || n <= 0) return 0;
+  return gram_layout(P, Q, n).total;
+}
+
+extern "C" int dftb_gram(const double* X, long ldx, int P, const double* Y, long ldy, int Q, long n, double* out,
+                         long ldo, void* workspace, size_t ws_bytes, void* stream) {
+  if (P <= 0 || Q <= 0) return DFTB_OK;
+  if (!X || !Y || !out || n <= 0 || ldx < n || ldy < n || ldo < Q || !workspace) return DFTB_ERR_ARG;
+  OrthoLayout o = gram_layout(P, Q, n);
+  if (ws_bytes < o.total) return DFTB_ERR_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  GemmArgs g = gemm_args_default();
+  g.A = X; g.lda = ldx;
+  g.B = Y; g.ldb = ldy; g.b_trans = 1;
+  g.C = workspace; g.ldc = Q; g.sCk = (long)P * Q;
+  g.M = P; g.N = Q; g.K = (int)n;
+  g.ksplit = o.ks; g.kchunk = (int)o.kchunk; g.tile = 64;
+  int rc = gemm_launch(g, st);
+  if (rc) return rc;
+  const long PQ = (long)P * Q;
+  long blocks = (PQ + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (const double*)workspace, o.ks, P,
+                     Q, out, ldo);
+  DFTB_CHECK_LAUNCH();
+  return DFTB_OK;
 }
 
 // ref[j] = sqrt(G_jj)

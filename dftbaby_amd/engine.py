@@ -101,24 +101,29 @@ def gemm(A, B, C, M, N, K, lda, ldb, ldc, a_trans=0, b_trans=0, alpha=1.0, beta=
 
 
 def gram(X, P, Y, Q, n, out, ws, ldx=None, ldy=None, ldo=None, x_off=0, y_off=0, o_off=0):
-    """out[p][q] = sum_e X[p][e] Y[q][e]  (tall-skinny, split-K, deterministic)."""
+    """out[p][q] = sum_e X[p][e] Y[q][e]  (tall-skinny, split-K and an in-order
+    on-device reduction: dftb_gram)."""
     ldx = ldx or n
     ldy = ldy or n
     ldo = ldo or Q
     if P == 0 or Q == 0:
         return
-    # enough K slices to fill the GPU (>= ~2048 workgroups of 64 x 64 tiles) with chunks of
-    # >= 2048 elements, so the small operand's chunk stays in L2 across the row tiles
-    tiles = ((P + 63) // 64) * ((Q + 63) // 64)
-    ks = max(1, min(512, (n + 2047) // 2048, (2048 + tiles - 1) // tiles))
-    kchunk = ((n + ks - 1) // ks + 31) // 32 * 32
-    ks = (n + kchunk - 1) // kchunk
-    part, _ = ws.get(ks * P * Q * 8)
-    part = part[:ks * P * Q * 8].view(torch.float64)
-    gemm(X, Y, part, P, Q, n, ldx, ldy, Q, b_trans=1, ksplit=ks, kchunk=kchunk, sCk=P * Q, tile=64,
-         A_off=x_off, B_off=y_off)
-    red = part[:ks * P * Q].view(ks, P, Q).sum(0) if ks > 1 else part[:P * Q].view(P, Q)
-    out.view(-1).as_strided((P, Q), (ldo, 1), o_off).copy_(red)
+    w, wn = ws.get(_lib.load().dftb_gram_workspace(int(P), int(Q), int(n)))
+    call("dftb_gram", ctypes.c_void_p(X.data_ptr() + 8 * x_off), int(ldx), int(P),
+         ctypes.c_void_p(Y.data_ptr() + 8 * y_off), int(ldy), int(Q), int(n),
+         ctypes.c_void_p(out.data_ptr() + 8 * o_off), int(ldo), _p(w), wn, _stream())
+
+
+def row_norms(X, k, n, norms, ws):
+    """norms[j] = ||X[j]||_2 for the k rows of X (length n)."""
+    w, wn = ws.get(_lib.load().dftb_norms_workspace(int(k)))
+    call("dftb_row_norms", _p(X), int(k), int(n), _p(norms), _p(w), wn, _stream())
+
+
+def residual(P, X, w, k, n, W, norms, ws):
+    """W[j] = P[j] - w[j] X[j] and norms[j] = ||W[j]|| (j < k)."""
+    wsb, wn = ws.get(_lib.load().dftb_norms_workspace(int(k)))
+    call("dftb_residual", _p(P), _p(X), _p(w), int(k), int(n), _p(W), _p(norms), _p(wsb), wn, _stream())
 
 
 def gemm_timing(on):
@@ -652,7 +657,7 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
     else:
         G = torch.cat([op.to_padded(XmYguess), op.to_padded(XpYguess)[:lmax - k]], 0)
         nrm = torch.empty(lmax, dtype=torch.float64, device=dev)
-        call("dftb_row_norms", _p(G), lmax, N, _p(nrm), st)
+        row_norms(G, lmax, N, nrm, ws)
         inv = 1.0 / nrm
         call("dftb_scale2", _p(G), _p(inv), None, lmax, N, _p(bs), st)
         lmax = _orthonormalize_block(bs, 0, lmax, N, ws, st)
@@ -665,10 +670,11 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
     Tb = torch.zeros((cap, k), dtype=torch.float64, device=dev)
     Rb = torch.zeros((cap, k), dtype=torch.float64, device=dev)
     Lb = torch.zeros((cap, k), dtype=torch.float64, device=dev)
-    RL = torch.empty((2 * k, N), dtype=torch.float64, device=dev)
-    R, L = RL[:k], RL[k:]
-    PR = torch.empty((k, N), dtype=torch.float64, device=dev)
-    ML = torch.empty((k, N), dtype=torch.float64, device=dev)
+    LR = torch.empty((2 * k, N), dtype=torch.float64, device=dev)          # [L; R] = [X-Y; X+Y]
+    L, R = LR[:k], LR[k:]
+    PM = torch.empty((2 * k, N), dtype=torch.float64, device=dev)          # [(A+B) R; (A-B) L]
+    PR, ML = PM[:k], PM[k:]
+    T2 = None                                                              # products of a new block
     W = torch.empty((2 * k, N), dtype=torch.float64, device=dev)
     norms = torch.empty(2 * k, dtype=torch.float64, device=dev)
     keep = torch.empty(2 * k, dtype=torch.int32, device=dev)
@@ -679,15 +685,23 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
         if XpYguess is None or it > 0:
             m = l - new0
             if m > 0:
+                if T2 is None or T2.shape[0] < 2 * m:
+                    T2 = torch.empty((2 * m, N), dtype=torch.float64, device=dev)
                 ev = stats.event_pair()
-                op.matvec(bs[new0:l], precision, out_p=bp[new0:l], out_m=bm[new0:l])
+                op.matvec(bs[new0:l], precision, out_p=T2[:m], out_m=T2[m:2 * m])
                 ev[1].record()
                 stats.matvec_vectors += m
-                # new rows and columns of M+- = bs^T (A+-B) bs
-                for P, M in ((bp, Mp), (bm, Mm)):
-                    gram(bs, l, P[new0:l], m, N, M, ws, ldo=cap, o_off=new0)          # M[:l, new]
-                    if new0 > 0:
-                        gram(bs[new0:l], m, P, new0, N, M, ws, ldo=cap, o_off=new0 * cap)  # M[new, :new0]
+                # new columns of M+- = bs^T (A+-B) bs from one pass over the basis;
+                # the new rows left of them by symmetry of A+-B
+                G = torch.empty((l, 2 * m), dtype=torch.float64, device=dev)
+                gram(bs, l, T2, 2 * m, N, G, ws)
+                Mp[:l, new0:l] = G[:, :m]
+                Mm[:l, new0:l] = G[:, m:]
+                if new0 > 0:
+                    Mp[new0:l, :new0] = Mp[:new0, new0:l].T
+                    Mm[new0:l, :new0] = Mm[:new0, new0:l].T
+                bp[new0:l].copy_(T2[:m])
+                bm[new0:l].copy_(T2[m:2 * m])
             if precision > 0:
                 # low-precision products: use the symmetric part of M+-, sanity-check the asymmetry
                 for Mx in (Mp, Mm):
@@ -708,23 +722,20 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
             if dg[1] > 1.0e-3:
                 raise ExcitedStatesError("(X+Y) and (X-Y) vectors not orthonormal! error = %s" % dg[1])
             # R = bs^T Rb, L = bs^T Lb and their products from the stored ones
-            RLb = torch.cat([Rb[:l], Lb[:l]], 1)          # one pass over the basis for R and L
-            gemm(RLb, bs, RL, 2 * k, N, l, 2 * k, N, N, a_trans=1, tile=64)
+            LRb = torch.cat([Lb[:l], Rb[:l]], 1)          # one pass over the basis for L and R
+            gemm(LRb, bs, LR, 2 * k, N, l, 2 * k, N, N, a_trans=1, tile=64)
             gemm(Rb, bp, PR, k, N, l, k, N, N, a_trans=1, tile=64)
             gemm(Lb, bm, ML, k, N, l, k, N, N, a_trans=1, tile=64)
-            w_cur = wdev
         else:
             R.copy_(op.to_padded(XpYguess))
             L.copy_(op.to_padded(XmYguess))
-            both = torch.cat([R, L], 0)
-            p_, m_ = op.matvec(both, precision)
+            p_, m_ = op.matvec(LR, precision)
             stats.matvec_vectors += 2 * k
-            PR.copy_(p_[:k])
-            ML.copy_(m_[k:])
+            PR.copy_(p_[k:])
+            ML.copy_(m_[:k])
             wdev.copy_(torch.as_tensor(np.asarray(w_guess[:k], dtype=np.float64), device=dev))
-            w_cur = wdev
-        call("dftb_residual", _p(PR), _p(L), _p(w_cur), k, N, _p(W[:k]), _p(norms[:k]), st)
-        call("dftb_residual", _p(ML), _p(R), _p(w_cur), k, N, _p(W[k:]), _p(norms[k:]), st)
+        # WL = (A+B) R - w L and WR = (A-B) L - w R (Solver.py:809-812) in one pass
+        residual(PM, LR, torch.cat([wdev, wdev]), 2 * k, N, W, norms, ws)
         nr = norms.cpu().numpy()
         nL, nR = nr[:k], nr[k:]
         nsum = nL + nR
@@ -792,7 +803,7 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
             "Davidson-like diagonalization for non-Hermitian eigenvalue problem did not converge after %s "
             "iterations (threshold: %e) !!!" % (maxiter, conv))
     if refine_now:
-        V0 = torch.cat([L, R], 0).contiguous()
+        V0 = LR.clone()
         stats.refined = True
         out = nonhermitian_davidson_gpu(op, k, conv=conv, maxiter=max(maxiter - (it + 1), 10), precision=0,
                                         verbose=verbose, stats=stats, initial_space=V0, precond_omega=precond_omega,
@@ -838,7 +849,7 @@ def hermitian_davidson_gpu(op, nstates, omega_shift=None, ifact=1, conv=1.0e-14,
         G = op.to_padded(np.asarray(XpYguess)[:lmax])
         call("dftb_scale2", _p(G), None, _p(sq_inv), lmax, N, _p(G), st)
         nrm = torch.empty(lmax, dtype=torch.float64, device=dev)
-        call("dftb_row_norms", _p(G), lmax, N, _p(nrm), st)
+        row_norms(G, lmax, N, nrm, ws)
         call("dftb_scale2", _p(G), _p(1.0 / nrm), None, lmax, N, _p(bs), st)
         lmax = _orthonormalize_block(bs, 0, lmax, N, ws, st)
     l = lmax
@@ -855,9 +866,10 @@ def hermitian_davidson_gpu(op, nstates, omega_shift=None, ifact=1, conv=1.0e-14,
             op.hermitian_matvec(bs[new0:l], sq, om, precision, out=rbs[new0:l])
             ev[1].record()
             stats.matvec_vectors += m
+            # new columns of H = bs^T R bs; the new rows left of them by symmetry of R
             gram(bs, l, rbs[new0:l], m, N, H, ws, ldo=cap, o_off=new0)
             if new0 > 0:
-                gram(bs[new0:l], m, rbs, new0, N, H, ws, ldo=cap, o_off=new0 * cap)
+                H[new0:l, :new0] = H[:new0, new0:l].T
         nev = l if use_l2 else k
         w2 = torch.empty(nev, dtype=torch.float64, device=dev)
         Tb = torch.empty((l, nev), dtype=torch.float64, device=dev)
@@ -880,7 +892,7 @@ def hermitian_davidson_gpu(op, nstates, omega_shift=None, ifact=1, conv=1.0e-14,
             T = T[oi].contiguous()
             RT = RT[oi].contiguous()
         W = torch.empty((k, N), dtype=torch.float64, device=dev)
-        call("dftb_residual", _p(RT), _p(T), _p(w2), k, N, _p(W), _p(norms), st)
+        residual(RT, T, w2, k, N, W, norms, ws)
         nr = norms[:k].cpu().numpy()
         w_host = np.sqrt(w2[:k].cpu().numpy())
         stats.history.append((l, nr.copy(), w_host.copy()))
@@ -1091,7 +1103,7 @@ def zvector_gpu(op, B, X0=None, maxiter=1000, conv=1.0e-14, precision=0, verbose
         gemm(Xb, bs, X, k, N, l, k, N, N, a_trans=1)
         gemm(Xb, P, W, k, N, l, k, N, N, a_trans=1)              # (A+B) X from the stored products
         W.sub_(Bd)
-        call("dftb_row_norms", _p(W), k, N, _p(norms), st)
+        row_norms(W, k, N, norms, ws)
         nr = norms.cpu().numpy()
         if verbose > 0:
             print("Iteration %d: ( %d expansion vectors )   time: %2.6f s" % (it, l, time.time() - t0))

@@ -192,15 +192,26 @@ int dftb_convert_bf16pair_atom_major(const double* q, long np, long nppad, int n
 
 /* ---- (c) Davidson support -------------------------------------------------
  * Residuals and preconditioner of Solver.py:809-884 / 480-530.
- *   W[j] = P[j] - w[j] X[j];  norms[j] = ||W[j]||_2            (j < k, length n)
+ *   W[j] = P[j] - w[j] X[j];  norms[j] = ||W[j]||_2            (j < k, length n;
+ *                                   workspace >= dftb_norms_workspace(k) bytes)
  *   Q[t] = W[sel[t]] / (w[sel[t]] - omega) with |w - omega| < 1e-6 replaced by
  *          1e-6 * omega (Solver.py:865-868)                     (t < nsel)
  * w, sel and norms live in device memory. */
+size_t dftb_norms_workspace(int k);
 int dftb_residual(const double* P, const double* X, const double* w, int k, long n,
-                  double* W, double* norms, void* stream);
+                  double* W, double* norms, void* workspace, size_t ws_bytes, void* stream);
 int dftb_precondition(const double* W, const double* w, const int* sel, int nsel,
                       const double* omega, long n, double* Q, void* stream);
-int dftb_row_norms(const double* X, int k, long n, double* norms, void* stream);
+int dftb_row_norms(const double* X, int k, long n, double* norms, void* workspace, size_t ws_bytes,
+                   void* stream);
+/* Projections of the Davidson subspace matrices (M+- = bs^T (A+-B) bs,
+ * Solver.py:757-761, and the tensordot overlaps of 890-897):
+ *   out[p * ldo + q] = sum_{e < n} X[p * ldx + e] Y[q * ldy + e]   (p < P, q < Q)
+ * split over n (workspace >= dftb_gram_workspace(P, Q, n) bytes), partial sums
+ * added in a fixed order (deterministic). */
+size_t dftb_gram_workspace(int P, int Q, long n);
+int dftb_gram(const double* X, long ldx, int P, const double* Y, long ldy, int Q, long n, double* out, long ldo,
+              void* workspace, size_t ws_bytes, void* stream);
 /* X[j] *= s[j] */
 int dftb_scale_rows(double* X, const double* s, int k, long n, void* stream);
 

@@ -64,6 +64,36 @@ def test_gram_narrow_splitk(rng):
         np.testing.assert_allclose(out.cpu().numpy(), X @ Y.T, rtol=1e-11, atol=1e-9)
 
 
+def test_gram_strided_output_and_residual_norms(rng):
+    """dftb_gram into a strided block of a larger matrix (the Davidson's new
+    columns of M), and the sliced residual / row-norm kernels (deterministic:
+    two calls give identical bits) against numpy."""
+    from dftbaby_amd.engine import gram, residual, row_norms, Workspace
+    ws = Workspace(_dev())
+    l, m, n = 230, 80, 200003
+    X = rng.standard_normal((l, n))
+    Y = rng.standard_normal((m, n))
+    M = torch.full((300, 400), 7.0, dtype=torch.float64, device=_dev())
+    gram(T(X), l, T(Y), m, n, M, ws, ldo=400, o_off=5 * 400 + 11)
+    Mh = M.cpu().numpy()
+    np.testing.assert_allclose(Mh[5:5 + l, 11:11 + m], X @ Y.T, rtol=1e-11, atol=1e-9)
+    assert np.all(Mh[:5] == 7.0) and np.all(Mh[5:5 + l, :11] == 7.0) and np.all(Mh[5:5 + l, 11 + m:] == 7.0)
+    k = 40
+    P, V = rng.standard_normal((k, n)), rng.standard_normal((k, n))
+    w = rng.standard_normal(k)
+    W = torch.empty((k, n), dtype=torch.float64, device=_dev())
+    nr = torch.empty(k, dtype=torch.float64, device=_dev())
+    residual(T(P), T(V), T(w), k, n, W, nr, ws)
+    ref = P - w[:, None] * V
+    np.testing.assert_allclose(W.cpu().numpy(), ref, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(nr.cpu().numpy(), np.linalg.norm(ref, axis=1), rtol=1e-13)
+    a = nr.cpu().numpy().copy()
+    residual(T(P), T(V), T(w), k, n, W, nr, ws)
+    assert np.array_equal(a, nr.cpu().numpy())
+    row_norms(T(V), k, n, nr, ws)
+    np.testing.assert_allclose(nr.cpu().numpy(), np.linalg.norm(V, axis=1), rtol=1e-13)
+
+
 def test_gemm_f64_batched_remap_splitk(rng):
     from dftbaby_amd.engine import gemm
     from dftbaby_amd import _lib
