@@ -293,6 +293,7 @@ int gemm_launch(const GemmArgs& p0, hipStream_t st, bool timed) {
     else if (small && p.M <= 48 && p.N > 48) launch_fp<double, 48, 64, 16, 1>(p, st);
     else if (small && p.N <= 32 && p.M > 48) launch_fp<double, 64, 32, 16, 4>(p, st);
     else if (small && p.N <= 48 && p.M > 48) launch_fp<double, 64, 48, 16, 4>(p, st);
+    else if (small && p.N <= 80 && p.M > 80) launch_fp<double, 64, 80, 16, 4>(p, st);   // both Gram halves of a block
     else if (small) launch_fp<double, 64, 64, 16>(p, st);
     else launch_fp<double, 128, 128, 16>(p, st);
   } else {

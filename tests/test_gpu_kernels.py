@@ -64,6 +64,28 @@ def test_gram_narrow_splitk(rng):
         np.testing.assert_allclose(out.cpu().numpy(), X @ Y.T, rtol=1e-11, atol=1e-9)
 
 
+@pytest.mark.parametrize("M,N,K,at", [(5, 20001, 7, 1), (20, 50000, 33, 1), (40, 50000, 100, 1), (64, 20001, 70, 0),
+                                       (24, 50000, 5, 0), (33, 40000, 129, 1)])
+def test_skinny_output_products(M, N, K, at, rng):
+    """C = alpha op(A) B + beta C with M <= 64 rows and a long N (the basis
+    products of the Davidson loop), batch 2 with strides."""
+    from dftbaby_amd import _lib
+    A = rng.standard_normal((2, K, M) if at else (2, M, K))
+    B = rng.standard_normal((2, K, N))
+    C0 = rng.standard_normal((2, M, N))
+    Ad, Bd, Cd = T(A), T(B), T(C0)
+    g = _lib.GemmArgs()
+    g.A, g.B, g.C = Ad.data_ptr(), Bd.data_ptr(), Cd.data_ptr()
+    g.lda, g.ldb, g.ldc = (M if at else K), N, N
+    g.sA, g.sB, g.sC = A[0].size, B[0].size, C0[0].size
+    g.cs2, g.M, g.N, g.K, g.batch, g.ksplit, g.kchunk = 1, M, N, K, 2, 1, 0
+    g.alpha, g.beta, g.a_trans, g.hmod = -0.5, 2.0, at, 1
+    _lib.call("dftb_gemm", ctypes.byref(g), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    for b in range(2):
+        ref = -0.5 * ((A[b].T if at else A[b]) @ B[b]) + 2.0 * C0[b]
+        np.testing.assert_allclose(Cd[b].cpu().numpy(), ref, rtol=1e-12, atol=1e-11)
+
+
 def test_gram_strided_output_and_residual_norms(rng):
     """dftb_gram into a strided block of a larger matrix (the Davidson's new
     columns of M), and the sliced residual / row-norm kernels (deterministic:
