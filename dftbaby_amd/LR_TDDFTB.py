@@ -47,6 +47,10 @@ class LR_TDDFTB:
     # long-range exchange products: "ao" (AO-basis GEMMs) or "charges" (the
     # reference's gamma_lr-folded transition-charge contractions); same results
     exchange = "ao"
+    # True: the launch shapes of the (A+-B) products are picked by the project's
+    # autotuner (dftbaby_amd.autotune: measured once per operator size and
+    # precision, cached) before the Davidson
+    autotune = False
 
     def __init__(self, atomlist=None, ground_state=None, **opts):
         if ground_state is None:
@@ -256,6 +260,9 @@ class LR_TDDFTB:
             omega_shift = LongRangeCTCorrection(self).diagonal_shifts(self.omega,
                                                                        verbose=getattr(self.dftb2, "verbose", 0))
         self.omega_shift = omega_shift
+        if self.autotune and nstates is not None:
+            from .autotune import autotune
+            self.tuning = autotune(op, prec)
         verbose = getattr(self.dftb2, "verbose", 0)
         t0 = time.time()
         if nstates is None:

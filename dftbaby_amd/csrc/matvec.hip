@@ -245,8 +245,11 @@ static int coulomb(const CasidaOperator* op, const MvLayout& L, char* ws, const 
 // (btrans = 1) or alpha A B (btrans = 0, FP64/FP32 only); final = FP64 output.
 static int ao_gemm(int prec, const void* A, const void* A2, long lda, const void* B, const void* B2, long ldb,
                    long sB, int btrans, void* C, void* C2, long ldc, long sC, int M, int N, int K, int batch,
-                   double alpha, bool final, const CasidaOperator* op, bool had, hipStream_t st) {
+                   double alpha, bool final, const CasidaOperator* op, bool had, hipStream_t st, int tile = 0) {
   GemmArgs g = gemm_args_default();
+  // a tuned pipelined tile (256: 256 x 128, 512: 256 x 256) applies where the
+  // automatic choice is the pipelined kernel too
+  g.tile = (prec == 2 && (tile == 256 || tile == 512) && M >= 256 && N >= 128 && K >= 128) ? tile : 0;
   g.in_type = prec == 0 ? GEMM_IN_F64 : (prec == 1 ? GEMM_IN_F32 : GEMM_IN_BF16X3);
   g.out_type = final ? GEMM_OUT_F64 : (prec == 0 ? GEMM_OUT_F64 : (prec == 1 ? GEMM_OUT_F32 : GEMM_OUT_BF16PAIR));
   g.A = A; g.A2 = A2; g.lda = lda; g.sA = 0;
@@ -296,7 +299,7 @@ static int ao_exchange(const CasidaOperator* op, const MvLayout& L, char* ws, in
     double* x2 = X2 + (size_t)n0 * N;
     // ---- K1: B1 = [Y_v; C_v] V^T, BH = G o ([Y_o; C_o] B1^T), B3 = [C_v; Y_v]^T BH^T, X1 = 1/4 W B3^T
     if ((rc = ao_gemm(prec, op->ao_v[0], op->ao_v[1], vpad, v0, v0l, vpad, N, 1, b1, b1l, opad, e1, (int)np2,
-                      (int)opad, (int)vpad, c, 1.0, false, op, false, st))) return rc;
+                      (int)opad, (int)vpad, c, 1.0, false, op, false, st, op->tile_expand))) return rc;
     if (prec == 2) {
       // BH and BH^T (= the K2 bracket) for the c vectors, then B3 of both in one launch
       GemmArgs g = gemm_args_default();
@@ -308,14 +311,15 @@ static int ao_exchange(const CasidaOperator* op, const MvLayout& L, char* ws, in
       g.Ct = bh + (size_t)c * eh * es; g.Ct2 = bhl + (size_t)c * eh * es; g.ldct = np2; g.sCt = eh;
       g.M = (int)np2; g.N = (int)np2; g.K = (int)opad; g.batch = c; g.alpha = 1.0;
       g.had = op->gam_ao; g.hmod = op->npad; g.ldh = op->nat + 1; g.hmap = op->amap;
+      g.tile = (op->tile_expand == 256 || op->tile_expand == 512) ? op->tile_expand : 0;
       if ((rc = gemm_launch(g, st))) return rc;
       if ((rc = ao_gemm(prec, op->ao_cyvt[0], op->ao_cyvt[1], np2, bh, bhl, np2, eh, 1, b3, b3l, np2, e3,
-                        (int)vpad, (int)np2, (int)np2, 2 * c, 1.0, false, op, false, st))) return rc;
+                        (int)vpad, (int)np2, (int)np2, 2 * c, 1.0, false, op, false, st, op->tile_contract))) return rc;
       if ((rc = ao_gemm(prec, op->ao_w[0], op->ao_w[1], np2, b3, b3l, np2, e3, 1, x1, nullptr, vpad, N, (int)opad,
-                        (int)vpad, (int)np2, c, 0.25, true, op, false, st))) return rc;
+                        (int)vpad, (int)np2, c, 0.25, true, op, false, st, op->tile_contract))) return rc;
       const size_t o3 = (size_t)c * e3 * es;
       if ((rc = ao_gemm(prec, op->ao_w[0], op->ao_w[1], np2, b3 + o3, b3l + o3, np2, e3, 1, x2, nullptr, vpad, N,
-                        (int)opad, (int)vpad, (int)np2, c, 0.25, true, op, false, st))) return rc;
+                        (int)opad, (int)vpad, (int)np2, c, 0.25, true, op, false, st, op->tile_contract))) return rc;
       continue;
     }
     if ((rc = ao_gemm(prec, op->ao_o[0], op->ao_o[1], opad, b1, b1l, opad, e1, 1, bh, bhl, np2, eh, (int)np2,

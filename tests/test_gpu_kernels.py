@@ -675,3 +675,35 @@ def test_orthonormalize(rng):
     for row in Qh[k == 1]:
         res = row - basis.T @ (basis @ row)
         assert np.linalg.norm(res) < 1e-9 * np.linalg.norm(row)
+
+
+def test_autotune_ao_tiles_are_used(tmp_path):
+    """The project autotuner times the BF16x3 AO chain's pipelined tiles on full
+    block matvecs, stores its pick in op.tuning (passed to every later matvec)
+    and the cache; an operator of the same size takes it from the cache without
+    measuring; tuned products equal the FP64 products to the BF16x3 tolerance;
+    LR_TDDFTB(autotune) runs it before the Davidson."""
+    from dftbaby_amd.autotune import autotune
+    from dftbaby_amd.LR_TDDFTB import LR_TDDFTB
+    op, _ = _operator("C180H36", 1, exchange="ao")
+    cache = str(tmp_path / "tuning.json")
+    lines = []
+    cfg = autotune(op, 2, nvec=8, cache=cache, log=lines.append)
+    assert len(lines) == 4 and cfg["tile_expand"] in (256, 512) and cfg["tile_contract"] in (256, 512)
+    assert op.tuning[2] == cfg
+    s = op._struct(2)
+    assert (s.tile_expand, s.tile_contract) == (cfg["tile_expand"], cfg["tile_contract"])
+    op2, _ = _operator("C180H36", 1, exchange="ao")
+    lines2 = []
+    assert autotune(op2, 2, nvec=8, cache=cache, log=lines2.append) == cfg and not lines2
+    V = torch.zeros((6, op.N), dtype=torch.float64, device=_dev())
+    V.view(6, op.opad, op.vpad)[:, :op.nocc, :op.nvirt] = torch.randn((6, op.nocc, op.nvirt), dtype=torch.float64,
+                                                                      device=_dev())
+    P2, M2 = op.matvec(V, 2)
+    P0, M0 = op.matvec(V, 0)
+    for a, b in ((P2, P0), (M2, M0)):
+        assert float((a - b).abs().max()) < 3e-5 * float(b.abs().max())
+    td = LR_TDDFTB(ground_state=ground_state("C12H14", 1, 0.05, 3))
+    td.autotune = True
+    td.getEnergies(nstates=4, diag_conv=1e-6, diag_maxiter=100, precision="bf16x3")
+    assert td.op.tuning.get(2)
