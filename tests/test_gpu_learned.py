@@ -28,7 +28,8 @@ def test_mlp_forward_backward_match_cpu(rng):
 
 
 def test_learned_start_converges_to_plain_roots():
-    """The learned initial space changes the trajectory, not the converged roots."""
+    """The learned initial space changes the trajectory, not the converged roots,
+    and on a held-out C12H14 it converges in fewer iterations than the plain start."""
     from dftbaby_amd.engine import CasidaOperatorGPU, nonhermitian_davidson_gpu
     from dftbaby_amd.learned import PairMLP, learned_start, training_rows
     from oracle import tddftb_oracle as O
@@ -57,3 +58,6 @@ def test_learned_start_converges_to_plain_roots():
     w1, _, _, _, st1 = nonhermitian_davidson_gpu(op, k, conv=1e-7, maxiter=100, initial_space=V, precond_omega=pom)
     assert np.abs(w0.cpu().numpy() - w1.cpu().numpy()).max() < 1e-8
     print("plain iterations", st0.iterations, "learned iterations", st1.iterations)
+    # what the trained network gains on this held-out geometry: fewer Davidson
+    # iterations than the plain start (19 -> 15 measured)
+    assert st1.iterations < st0.iterations
