@@ -349,65 +349,34 @@ class LR_TDDFTB:
 
     # ----------------------------------------------------------- properties
     def _TransitionDipoles(self):
-        """d_I = sum_ia C^I_ia <i|r|a> (LR_TDDFTB.py:574-592).
-
-        A ground state that exposes getOrbitalCenters (the bundled producer of
-        dftbaby_amd.groundstate) has the Mulliken point-charge dipole matrix
-        D^c = S o (R^c_m + R^c_n) / 2 (R: centre of each AO), so <p|r_c|q> =
-        (P_c^T Q + Q^T P_c) / 2 with Q = S C and P_c = diag(R^c) C: two GEMMs per
-        component for the active occupied x virtual block, the full MO matrices
-        tdipMO_X/Y/Z formed on first access.  Any other ground state (DFTBaby's
-        own, whose getDipoleMatrix comes from its Slater-Koster dipole tables,
-        DFTB2.py:2446-2452) goes through the dense C^T D C of
-        _TransitionDipolesDense."""
+        """d_I = sum_ia C^I_ia <i|r|a> with <p|r_c|q> = (C^T D_c C)_pq
+        (LR_TDDFTB.py:574-592) and the ground state's dipole matrix D
+        (getDipoleMatrix, DFTB2.py:2446-2452: Slater-Koster dipole tables).  The
+        active occupied x virtual block takes two GEMMs per component,
+        X = D_c C_v and C_o^T X; the full MO matrices tdipMO_X/Y/Z are formed on
+        first access.  A ground state with getDipoleMatrixDevice (the bundled
+        producer) hands over device planes [3][norb][norb] from the
+        dftb_slako_matrices kernel; any other one its numpy [norb][norb][3]."""
         op = self.op
         dev = op.device
         gs = self.dftb2
-        if not hasattr(gs, "getOrbitalCenters"):
-            return self._TransitionDipolesDense()
-        S = torch.as_tensor(np.ascontiguousarray(gs.getOverlapMatrix(), dtype=np.float64), device=dev)
+        if hasattr(gs, "getDipoleMatrixDevice"):
+            Dp = gs.getDipoleMatrixDevice(dev)
+        else:
+            D = np.asarray(gs.getDipoleMatrix(), dtype=np.float64)
+            Dp = torch.as_tensor(np.ascontiguousarray(np.moveaxis(D, 2, 0)), device=dev)
         C = torch.as_tensor(np.ascontiguousarray(gs.getKSCoefficients(), dtype=np.float64), device=dev)
-        R = torch.as_tensor(np.ascontiguousarray(gs.getOrbitalCenters(), dtype=np.float64), device=dev)
         norb = C.shape[0]
         o, v = op.nocc, op.nvirt
         Co = C[:, torch.as_tensor(self.active_occupied_orbs, device=dev)].contiguous()
         Cv = C[:, torch.as_tensor(self.active_virtual_orbs, device=dev)].contiguous()
-        Qo = torch.empty_like(Co)
-        Qv = torch.empty_like(Cv)
-        gemm(S, Co, Qo, norb, o, norb, norb, o, o)
-        gemm(S, Cv, Qv, norb, v, norb, norb, v, v)
         M = torch.zeros((3, op.opad, op.vpad), dtype=torch.float64, device=dev)
+        X = torch.empty((norb, v), dtype=torch.float64, device=dev)
         for c in range(3):
-            Po = (R[:, c:c + 1] * Co).contiguous()
-            Pv = (R[:, c:c + 1] * Cv).contiguous()
-            gemm(Po, Qv, M[c], o, v, norb, o, v, op.vpad, a_trans=1, alpha=0.5)
-            gemm(Qo, Pv, M[c], o, v, norb, o, v, op.vpad, a_trans=1, alpha=0.5, beta=1.0)
-        self._tdipMO_src = (S, C, R)
+            gemm(Dp[c], Cv, X, norb, v, norb, norb, v, v)
+            gemm(Co, X, M[c], o, v, norb, o, v, op.vpad, a_trans=1)
+        self._tdipMO_src = (Dp, C)
         self._tdipMO_h = [None, None, None]
-        self._tdip_from(M.reshape(3, -1))
-        return self.tdipX, self.tdipY, self.tdipZ
-
-    def _TransitionDipolesDense(self):
-        """Same with the ground state's dense dipole matrix (getDipoleMatrix)."""
-        op = self.op
-        dev = op.device
-        D = np.asarray(self.dftb2.getDipoleMatrix(), dtype=np.float64)
-        orbs = np.asarray(self.dftb2.getKSCoefficients(), dtype=np.float64)
-        norb = orbs.shape[0]
-        Ct = torch.as_tensor(np.ascontiguousarray(orbs), device=dev)
-        M = torch.zeros((3, op.opad, op.vpad), dtype=torch.float64, device=dev)
-        ao = torch.as_tensor(self.active_occupied_orbs, device=dev)
-        av = torch.as_tensor(self.active_virtual_orbs, device=dev)
-        self._tdipMO_src = None
-        self._tdipMO_h = [None, None, None]
-        for c in range(3):
-            Dc = torch.as_tensor(np.ascontiguousarray(D[:, :, c]), device=dev)
-            X = torch.empty((norb, norb), dtype=torch.float64, device=dev)
-            gemm(Dc, Ct, X, norb, norb, norb, norb, norb, norb)
-            T = torch.empty((norb, norb), dtype=torch.float64, device=dev)
-            gemm(Ct, X, T, norb, norb, norb, norb, norb, norb, a_trans=1)
-            M[c, :op.nocc, :op.nvirt] = T[ao][:, av]
-            self._tdipMO_h[c] = T.cpu().numpy()
         self._tdip_from(M.reshape(3, -1))
         return self.tdipX, self.tdipY, self.tdipZ
 
@@ -417,14 +386,12 @@ class LR_TDDFTB:
         if h is None:
             raise AttributeError("transition dipoles between MOs are formed by getEnergies(oszis='Dipoles')")
         if h[c] is None:
-            S, C, R = self._tdipMO_src
+            Dp, C = self._tdipMO_src
             norb = C.shape[0]
-            Q = torch.empty_like(C)
-            gemm(S, C, Q, norb, norb, norb, norb, norb, norb)
-            P = (R[:, c:c + 1] * C).contiguous()
+            X = torch.empty_like(C)
+            gemm(Dp[c], C, X, norb, norb, norb, norb, norb, norb)
             T = torch.empty_like(C)
-            gemm(P, Q, T, norb, norb, norb, norb, norb, norb, a_trans=1, alpha=0.5)
-            gemm(Q, P, T, norb, norb, norb, norb, norb, norb, a_trans=1, alpha=0.5, beta=1.0)
+            gemm(C, X, T, norb, norb, norb, norb, norb, norb, a_trans=1)
             h[c] = T.cpu().numpy()
         return h[c]
 

@@ -1,0 +1,220 @@
+// Slater-Koster assembly of the overlap S, the reference hamiltonian H0 and the
+// dipole matrix D between valence orbitals (s and p), for the ground-state
+// producer.  Restates DFTB/extensions/slako.f90:77-394 (H0andS, DipoleMatrix)
+// with the B-spline evaluation of splines.f90:25-94 (splev_uniform), including
+// its interval rule l = INT((x - t(1)) / dt) + 1 and its clamping at the table
+// ends (see dftb_slako_matrices in include/dftbaby_amd.h).
+//
+// One thread per output element (row, col): it forms the lower-triangle element
+// (max, min) of the pair, as the reference does before mirroring, so the two
+// triangles are bit-identical and every row is written by consecutive threads.
+// The tables (one knot vector per atom-type pair, 29 coefficient rows: S, H0,
+// dipole) are a few hundred kB and stay in L2; the kernel is bound by its HBM
+// writes (40 bytes per element with D).
+#include "common.h"
+#include "dftbaby_amd.h"
+
+namespace {
+
+constexpr int kDeg = 3;   // cubic B-splines (splrep's default degree)
+
+// B-spline basis and knot interval for one distance, as splev_uniform/fpbspl.
+struct Basis {
+  double h[kDeg + 1];
+  int c0;   // first coefficient index (0-based)
+};
+
+__device__ Basis spline_basis(const double* __restrict__ t, int n, double x) {
+  const int k1 = kDeg + 1, nk1 = n - k1;
+  const double tb = t[k1 - 1], te = t[nk1];
+  double arg;
+  int l;   // 1-based knot interval, as in splines.f90
+  if (x <= tb) {
+    arg = tb;
+    l = k1;
+  } else if (x >= te) {
+    arg = te;
+    l = nk1;
+  } else {
+    arg = x;
+    const double dt = t[k1 + 1] - t[k1];
+    l = (int)((x - t[0]) / dt) + 1;
+  }
+  // splines.f90 moves l < k to k+1; l == k (x within three grid steps of the
+  // first point) would read before the coefficient array there.  Distinct atoms
+  // are never that close; clamp so the read stays in bounds.
+  if (l < k1) l = k1;
+  if (l > nk1) l = nk1;
+  Basis b;
+  b.h[0] = 1.0;
+#pragma unroll
+  for (int j = 1; j <= kDeg; ++j) {
+    double hh[kDeg];
+#pragma unroll
+    for (int i = 0; i < j; ++i) hh[i] = b.h[i];
+    b.h[0] = 0.0;
+#pragma unroll
+    for (int i = 1; i <= j; ++i) {
+      const int li = l + i, lj = li - j;
+      const double f = hh[i - 1] / (t[li - 1] - t[lj - 1]);
+      b.h[i - 1] = b.h[i - 1] + f * (t[li - 1] - arg);
+      b.h[i] = f * (arg - t[lj - 1]);
+    }
+  }
+  b.c0 = l - k1;
+  return b;
+}
+
+__device__ __forceinline__ double spline_value(const double* __restrict__ c, const Basis& b) {
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j <= kDeg; ++j) s = s + c[b.c0 + j] * b.h[j];
+  return s;
+}
+
+// real p orbital m = -1, 0, 1 -> Cartesian y, z, x
+__device__ __forceinline__ int pidx(int m) { return m == -1 ? 1 : (m == 0 ? 2 : 0); }
+
+__device__ __forceinline__ double sum_other_sq(const double* e, int i) {
+  const double x2 = e[0] * e[0], y2 = e[1] * e[1], z2 = e[2] * e[2];
+  return i == 0 ? y2 + z2 : (i == 1 ? x2 + z2 : x2 + y2);
+}
+
+// <first|second> from the table values T[0..6] (ss-sigma 0, sp-sigma 2,
+// ps-sigma 4, pp-pi 5, pp-sigma 6), e: unit vector first -> second.
+__device__ double sk_rule(int l1, int m1, int l2, int m2, const double* e, const double* T) {
+  if (l1 == 0 && l2 == 0) return T[0];
+  if (l1 == 0) return e[pidx(m2)] * T[2];
+  if (l2 == 0) return e[pidx(m1)] * T[4];
+  const int i = pidx(m1), j = pidx(m2);
+  if (i == j) return sum_other_sq(e, i) * T[5] + e[i] * e[i] * T[6];
+  return e[i] * e[j] * (T[6] - T[5]);
+}
+
+// <first|r_M|second> (without the origin shift) from the dipole tables D[0..14]
+// (indices 1, 3, 4, 6, 7, 8, 13, 14 carry data for s and p).
+__device__ double sk_rule_dipole(int l1, int m1, int M, int l2, int m2, const double* e, const double* D) {
+  if (l1 == 0 && l2 == 0) return e[M] * D[3];
+  if (l1 == 0) {
+    const int j = pidx(m2);
+    if (M == j) return sum_other_sq(e, j) * D[1] + e[j] * e[j] * D[4];
+    return e[M] * e[j] * (D[4] - D[1]);
+  }
+  if (l2 == 0) {
+    const int i = pidx(m1);
+    if (M == i) return sum_other_sq(e, i) * D[6] + e[i] * e[i] * D[13];
+    return e[i] * e[M] * (D[13] - D[6]);
+  }
+  const int i = pidx(m1), j = pidx(m2);
+  double v = (D[14] - D[7] - 2.0 * D[8]) * e[i] * e[M] * e[j];
+  if (i == M) v += D[7] * e[j];
+  if (M == j) v += D[8] * e[i];
+  if (i == j) v += D[8] * e[M];
+  return v;
+}
+
+__global__ void __launch_bounds__(256) slako_kernel(const double* __restrict__ pos, const int* __restrict__ orb_atom,
+                                                    const int* __restrict__ orb_l, const int* __restrict__ orb_m,
+                                                    const int* __restrict__ atom_type,
+                                                    const double* __restrict__ orben, int norb,
+                                                    const double* __restrict__ knots,
+                                                    const double* __restrict__ coefs, int npair, int npts,
+                                                    double cutoff, double* __restrict__ S, double* __restrict__ H,
+                                                    double* __restrict__ Dm) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.y;
+  if (col >= norb) return;
+  const int hi = row > col ? row : col, lo = row > col ? col : row;
+  const int A = orb_atom[hi], B = orb_atom[lo];
+  const size_t o = (size_t)row * norb + col;
+  const size_t plane = (size_t)norb * norb;
+  const double xc = pos[3 * B] - pos[3 * A], yc = pos[3 * B + 1] - pos[3 * A + 1], zc = pos[3 * B + 2] - pos[3 * A + 2];
+  const double r = sqrt(xc * xc + yc * yc + zc * zc);
+  double sv = 0.0, hv = 0.0, dv[3] = {0.0, 0.0, 0.0};
+  if (A == B || r < cutoff) {
+    const int tA = atom_type[A], tB = atom_type[B];
+    const bool swap = tA > tB;     // the orbital on the lighter type goes first
+    const int first = swap ? lo : hi, second = swap ? hi : lo;
+    const int tmin = swap ? tB : tA, tmax = swap ? tA : tB;
+    const int ip = tmax * (tmax + 1) / 2 + tmin;
+    const int l1 = orb_l[first], m1 = orb_m[first], l2 = orb_l[second], m2 = orb_m[second];
+    double e[3];
+    double rr;
+    if (A == B) {
+      e[0] = 0.0; e[1] = 0.0; e[2] = 1.0;
+      rr = 0.0;
+    } else {
+      e[0] = xc / r; e[1] = yc / r; e[2] = zc / r;
+      if (swap) { e[0] = -e[0]; e[1] = -e[1]; e[2] = -e[2]; }
+      rr = r;
+    }
+    if (ip < npair && l1 <= 1 && l2 <= 1) {
+      const double* t = knots + (size_t)ip * npts;
+      const double* c = coefs + (size_t)ip * SLAKO_NTAB * npts;
+      const Basis b = spline_basis(t, npts, rr);
+      if (A == B) {
+        if (hi == lo) { sv = 1.0; hv = orben[hi]; }
+      } else {
+        double Ts[7], Th[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) { Ts[k] = 0.0; Th[k] = 0.0; }
+        // only the tables the rule of (l1, l2) reads
+        const int need0 = (l1 == 0 && l2 == 0) ? 0 : (l1 == 0 ? 2 : (l2 == 0 ? 4 : 5));
+        const int need1 = (l1 == 1 && l2 == 1) ? 6 : need0;
+        Ts[need0] = spline_value(c + (size_t)need0 * npts, b);
+        Th[need0] = spline_value(c + (size_t)(7 + need0) * npts, b);
+        Ts[need1] = spline_value(c + (size_t)need1 * npts, b);
+        Th[need1] = spline_value(c + (size_t)(7 + need1) * npts, b);
+        sv = sk_rule(l1, m1, l2, m2, e, Ts);
+        hv = sk_rule(l1, m1, l2, m2, e, Th);
+      }
+      if (Dm) {
+        double Dt[15];
+#pragma unroll
+        for (int k = 0; k < 15; ++k) Dt[k] = 0.0;
+        if (l1 == 0 && l2 == 0) {
+          Dt[3] = spline_value(c + (size_t)(14 + 3) * npts, b);
+        } else if (l1 == 0) {
+          Dt[1] = spline_value(c + (size_t)(14 + 1) * npts, b);
+          Dt[4] = spline_value(c + (size_t)(14 + 4) * npts, b);
+        } else if (l2 == 0) {
+          Dt[6] = spline_value(c + (size_t)(14 + 6) * npts, b);
+          Dt[13] = spline_value(c + (size_t)(14 + 13) * npts, b);
+        } else {
+          Dt[7] = spline_value(c + (size_t)(14 + 7) * npts, b);
+          Dt[8] = spline_value(c + (size_t)(14 + 8) * npts, b);
+          Dt[14] = spline_value(c + (size_t)(14 + 14) * npts, b);
+        }
+        const int G = orb_atom[second];
+#pragma unroll
+        for (int M = 0; M < 3; ++M) dv[M] = sk_rule_dipole(l1, m1, M, l2, m2, e, Dt) + pos[3 * G + M] * sv;
+      }
+    } else {
+      sv = hv = dv[0] = dv[1] = dv[2] = __longlong_as_double(0x7ff8000000000000LL);   // unsupported input
+    }
+  }
+  S[o] = sv;
+  if (H) H[o] = hv;
+  if (Dm) {
+    Dm[o] = dv[0];
+    Dm[plane + o] = dv[1];
+    Dm[2 * plane + o] = dv[2];
+  }
+}
+
+}  // namespace
+
+extern "C" int dftb_slako_matrices(const double* pos, const int* orb_atom, const int* orb_l, const int* orb_m,
+                                   const int* atom_type, const double* orben, int nat, int norb,
+                                   const double* knots, const double* coefs, int npair, int npts, double cutoff,
+                                   double* S, double* H0, double* D, void* stream) {
+  if (nat <= 0 || norb <= 0 || npair <= 0 || npts < 2 * (kDeg + 1) + 2) return DFTB_ERR_ARG;
+  if (!pos || !orb_atom || !orb_l || !orb_m || !atom_type || !orben || !knots || !coefs || !S) return DFTB_ERR_ARG;
+  if (!(cutoff > 0.0)) return DFTB_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  dim3 grid(ceil_div(norb, 256), norb);
+  slako_kernel<<<grid, 256, 0, st>>>(pos, orb_atom, orb_l, orb_m, atom_type, orben, norb, knots, coefs, npair, npts,
+                                     cutoff, S, H0, D);
+  DFTB_CHECK_LAUNCH();
+  return DFTB_OK;
+}

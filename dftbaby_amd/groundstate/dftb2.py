@@ -8,14 +8,10 @@ the getters that ``DFTB/DFTB2.py:2405-2454`` exposes to ``LR_TDDFTB``
 ``Nelec_val``).  This module produces such a state for H/C molecules with the
 reference's "homegrown" parameters (converted by ``make_slako_data``):
 
-* H0 and S from the Slater-Koster tables with the two-centre s/p rules
-  (``DFTB/extensions/slako.f90:77-231``).  Table values are interpolated with
-  a not-a-knot cubic spline (the interpolant of ``scipy.interpolate.splrep``
-  with s=0) and are zero beyond the last grid point, as in the reference's
-  Python path (``DFTB/SlaterKoster/SKIntegrals.py:823-824``).  The reference's
-  default Fortran spline path (``splines.f90:75``) differs from this by ~1e-5
-  in S; the excited-state engine is checked on identical inputs, so this only
-  affects end-to-end comparisons with DFTBaby itself.
+* H0, S and the dipole matrix from the Slater-Koster tables, assembled as
+  DFTBaby's default Fortran path does (``DFTB/extensions/slako.f90:77-394`` with
+  the spline evaluation of ``splines.f90:25-94``), see ``groundstate/slako.py``;
+  on the GPU by the ``dftb_slako_matrices`` kernel.
 * gamma and gamma_lr for Gaussian charge fluctuations
   (``DFTB/GammaApproximation.py:8-82,122-151,210-243,388-405``), sigma = 1/(sqrt(pi) U).
 * SCC iterations with Mulliken charges and, for ``long_range_correction=1``,
@@ -24,10 +20,6 @@ reference's "homegrown" parameters (converted by ``make_slako_data``):
 * T=0 occupations (``DFTB/DFTB2.py:2594-2610``).
 * The Gaussian overlap Omega_AB for the Lambda2 diagnostic
   (``DFTB/DFTB2.py:1287-1317``).
-* The AO dipole matrix in the Mulliken point-charge approximation
-  D^c_{mu nu} = S_{mu nu} (R^c_A + R^c_B)/2.  The reference's default uses
-  Slater-Koster dipole tables; this producer does not, so oscillator strengths
-  computed from it equal the Mulliken-approximation ones.
 
 The repulsive potential is not included: excitation energies do not depend on
 it (``DFTB/DFTB2.py:100-102``).
@@ -37,8 +29,9 @@ import os
 
 import numpy as np
 import torch
-from scipy.interpolate import CubicSpline
 from scipy.special import erf
+
+from . import slako
 
 _DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "slako_hc.npz")
 
@@ -47,36 +40,17 @@ class SCFNotConverged(Exception):
     pass
 
 
-class _SlakoTables:
+class _Parameters:
+    """Hubbard U, valence orbitals, electron counts and free-atom populations of H and C."""
+
     def __init__(self):
         d = np.load(_DATA)
         self.U = {1: float(d["h_U"]), 6: float(d["c_U"])}
         # valence orbitals (n, l, m) per element, DFTB/DFTB2.py:240-245
         self.valorbs = {1: [(0, 0, 0)], 6: [(1, 0, 0), (1, 1, -1), (1, 1, 0), (1, 1, 1)]}
-        self.orbital_energies = {1: [float(d["h_energy_1s"])],
-                                 6: [float(d["c_energy_2s"])] + 3 * [float(d["c_energy_2p"])]}
         self.ne_val = {1: 1, 6: 4}
         # reference (neutral atom) populations per orbital
         self.occ_ref = {1: [1.0], 6: [2.0, 2.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0]}
-        self.splines = {}
-        self.dmax = {}
-        for pair, (z1, z2) in (("h_h", (1, 1)), ("h_c", (1, 6)), ("c_c", (6, 6))):
-            grid = d["%s_d" % pair]
-            self.dmax[(z1, z2)] = grid[-1]
-            for kind in ("S", "H"):
-                keys = d["%s_%s_keys" % (pair, kind)]
-                vals = d["%s_%s_vals" % (pair, kind)]
-                for k, v in zip(keys, vals):
-                    self.splines[(z1, z2, kind, tuple(int(x) for x in k))] = CubicSpline(grid, v)
-
-    def value(self, z1, z2, kind, key, r):
-        spl = self.splines.get((z1, z2, kind, key))
-        out = np.zeros_like(r)
-        if spl is None:
-            return out
-        inside = r <= self.dmax[(z1, z2)]
-        out[inside] = spl(r[inside])
-        return out
 
 
 _TABLES = None
@@ -85,39 +59,8 @@ _TABLES = None
 def _tables():
     global _TABLES
     if _TABLES is None:
-        _TABLES = _SlakoTables()
+        _TABLES = _Parameters()
     return _TABLES
-
-
-def _pair_integrals(T, kind, ZA, ZB, r):
-    """Slater-Koster integrals for atom pairs A->B (arrays over pairs).
-
-    Returns (ss, sp, ps, pp_sigma, pp_pi) where sp has s on A and p on B and
-    ps has p on A and s on B, in the orientation of the direction A->B.
-    """
-    n = len(r)
-    ss = np.zeros(n); sp = np.zeros(n); ps = np.zeros(n); pps = np.zeros(n); ppp = np.zeros(n)
-    for (za, zb) in ((1, 1), (1, 6), (6, 1), (6, 6)):
-        m = (ZA == za) & (ZB == zb)
-        if not m.any():
-            continue
-        rr = r[m]
-        if (za, zb) == (1, 1):
-            ss[m] = T.value(1, 1, kind, (0, 0, 0), rr)
-        elif (za, zb) == (1, 6):
-            ss[m] = T.value(1, 6, kind, (0, 0, 0), rr)
-            sp[m] = T.value(1, 6, kind, (0, 1, 2), rr)
-        elif (za, zb) == (6, 1):
-            ss[m] = T.value(1, 6, kind, (0, 0, 0), rr)
-            # p on C(A), s on H(B): E_{p_A s_B}(e) = E_{s_B p_A}(-e)
-            ps[m] = -T.value(1, 6, kind, (0, 1, 2), rr)
-        else:
-            ss[m] = T.value(6, 6, kind, (0, 0, 0), rr)
-            sp[m] = T.value(6, 6, kind, (0, 1, 2), rr)
-            ps[m] = T.value(6, 6, kind, (1, 0, 4), rr)
-            ppp[m] = T.value(6, 6, kind, (1, 1, 5), rr)
-            pps[m] = T.value(6, 6, kind, (1, 1, 6), rr)
-    return ss, sp, ps, pps, ppp
 
 
 class DFTB2:
@@ -151,47 +94,12 @@ class DFTB2:
         self.dim = int(self.orbsPerAtom.sum())
         self.Nelec_val = int(sum(self._T.ne_val[Z] for Z in self.Z))
         self.orb_atom = np.repeat(np.arange(len(self.Z)), self.orbsPerAtom)
-        for a in ("orbs", "orbe", "f", "S", "H0", "P"):
+        for a in ("orbs", "orbe", "f", "S", "H0", "P", "_D"):
             if hasattr(self, a):
                 delattr(self, a)
 
     def _constructH0andS(self):
-        T = self._T
-        nat = len(self.Z)
-        off = np.concatenate([[0], np.cumsum(self.orbsPerAtom)[:-1]])
-        S = np.zeros((self.dim, self.dim))
-        H0 = np.zeros((self.dim, self.dim))
-        for A in range(nat):
-            o = off[A]
-            for k, e in enumerate(T.orbital_energies[self.Z[A]]):
-                S[o + k, o + k] = 1.0
-                H0[o + k, o + k] = e
-        dvec = self.pos[None, :, :] - self.pos[:, None, :]
-        r = np.linalg.norm(dvec, axis=-1)
-        iA, iB = np.where((r > 1e-8) & (r < min(self.distance_cutoff, 10.5)))
-        rr = r[iA, iB]
-        e = dvec[iA, iB] / rr[:, None]
-        # real p orbitals: m=-1 -> y, m=0 -> z, m=+1 -> x
-        ecomp = np.stack([e[:, 1], e[:, 2], e[:, 0]], axis=1)
-        ZA, ZB = self.Z[iA], self.Z[iB]
-        oA, oB = off[iA], off[iB]
-        for M, kind in ((S, "S"), (H0, "H")):
-            ss, sp, ps, pps, ppp = _pair_integrals(T, kind, ZA, ZB, rr)
-            M[oA, oB] = ss
-            pB = ZB == 6
-            pA = ZA == 6
-            for c in range(3):
-                M[oA[pB], oB[pB] + 1 + c] = ecomp[pB, c] * sp[pB]
-                M[oA[pA] + 1 + c, oB[pA]] = ecomp[pA, c] * ps[pA]
-            both = pA & pB
-            for c1 in range(3):
-                for c2 in range(3):
-                    val = ecomp[both, c1] * ecomp[both, c2] * (pps[both] - ppp[both])
-                    if c1 == c2:
-                        val = val + ppp[both]
-                    M[oA[both] + 1 + c1, oB[both] + 1 + c2] = val
-        S = 0.5 * (S + S.T)
-        H0 = 0.5 * (H0 + H0.T)
+        S, H0, _ = slako.matrices_host(self.Z, self.pos, self.distance_cutoff, dipoles=False)
         return S, H0
 
     def _gamma_matrices(self):
@@ -224,12 +132,16 @@ class DFTB2:
         dev = self.device
         if dev is None:
             dev = "cuda" if (torch.cuda.is_available() and self.dim > 400) else "cpu"
-        S, H0 = self._constructH0andS()
-        self.S, self.H0 = S, H0
+        self._dev = dev
+        t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev)
+        if str(dev).startswith("cuda"):
+            St, H0t, _ = slako.matrices_device(self.Z, self.pos, dev, self.distance_cutoff, dipoles=False)
+            self.S, self.H0 = St.cpu().numpy(), H0t.cpu().numpy()
+        else:
+            self.S, self.H0 = self._constructH0andS()
+            St, H0t = t(self.S), t(self.H0)
         self.gamma, self.gamma_lr = self._gamma_matrices()
         self.gaussOmega = self._construct_gaussian_overlap()
-        t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev)
-        St, H0t = t(S), t(H0)
         oa = torch.as_tensor(self.orb_atom, device=dev)
         nat = len(self.Z)
         G = t(self.gamma)
@@ -329,15 +241,21 @@ class DFTB2:
     def getGaussianOverlap(self):
         return self.gaussOmega
 
-    def getOrbitalCenters(self):
-        """[norb][3] position of the atom carrying each AO (the dipole matrix below
-        is S_mn (R_m + R_n) / 2 with these centres)."""
-        return self.pos[self.orb_atom]
-
     def getDipoleMatrix(self):
-        S = self.getOverlapMatrix()
-        R = self.pos[self.orb_atom]
-        return S[:, :, None] * 0.5 * (R[:, None, :] + R[None, :, :])
+        """[norb][norb][3] dipole matrix from the Slater-Koster dipole tables
+        (DFTB2.py:2446-2452, slako.f90:238-394)."""
+        if getattr(self, "_D", None) is None:
+            dev = getattr(self, "_dev", "cpu")
+            if str(dev).startswith("cuda"):
+                self._D = np.ascontiguousarray(np.moveaxis(self.getDipoleMatrixDevice(dev).cpu().numpy(), 0, 2))
+            else:
+                self._D = slako.matrices_host(self.Z, self.pos, self.distance_cutoff)[2]
+        return self._D
+
+    def getDipoleMatrixDevice(self, dev):
+        """The dipole matrix as three device planes [3][norb][norb] (x, y, z),
+        formed by the dftb_slako_matrices kernel."""
+        return slako.matrices_device(self.Z, self.pos, dev, self.distance_cutoff)[2]
 
     def getGeometry(self):
         return self.atomlist

@@ -1,7 +1,8 @@
 """Convert DFTBaby's "homegrown" H/C parameter tables into one numpy data file.
 
 This is a build-time data step, not part of the solver.  It reads the
-Slater-Koster tables ``DFTB/SlaterKoster/slako_tables/{h_h,h_c,c_c}.py`` as
+Slater-Koster tables ``DFTB/SlaterKoster/slako_tables/{h_h,h_c,c_c}.py`` (overlap
+S, hamiltonian H and dipole integrals, keyed (l1, l2, table index)) as
 *data* (the files are generated modules holding ``array([...])`` literals; they
 are parsed with ``ast`` and never imported or executed) and the free
 pseudo-atom orbital energies of ``DFTB/SlaterKoster/free_pseudo_atoms/{h,c}.py``
@@ -43,10 +44,11 @@ def main(refroot):
     atdir = os.path.join(refroot, "DFTB", "SlaterKoster", "free_pseudo_atoms")
     data = {}
     for pair in ("h_h", "h_c", "c_c"):
-        t = parse_table(os.path.join(skdir, pair + ".py"), {"Z1", "Z2", "d", "S", "H"})
+        t = parse_table(os.path.join(skdir, pair + ".py"), {"Z1", "Z2", "d", "S", "H", "Dipole"})
         data["%s_Z" % pair] = np.array([t["Z1"], t["Z2"]], dtype=np.int64)
         data["%s_d" % pair] = t["d"]
-        for kind in ("S", "H"):
+        t["D"] = t["Dipole"]
+        for kind in ("S", "H", "D"):
             keys = sorted(t[kind].keys())
             data["%s_%s_keys" % (pair, kind)] = np.array(keys, dtype=np.int64)
             data["%s_%s_vals" % (pair, kind)] = np.stack([t[kind][k] for k in keys])

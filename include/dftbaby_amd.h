@@ -323,6 +323,27 @@ int dftb_adam(float* p, const float* g, float* m, float* v, long n, float lr, fl
               int step, float wd, void* stream);
 int dftb_mlp_normalize(float* X, long rows, int cols, const float* mean, const float* inv_std, void* stream);
 
+/* ---- ground-state inputs: Slater-Koster matrices ---------------------------
+ * S, H0 (Norb x Norb, row-major) and, if D != NULL, the dipole matrix as three
+ * planes D[c][mu][nu] (c = x, y, z) for s and p valence orbitals: replaces the
+ * f2py slako.h0ands and slako.dipolematrix (DFTB/extensions/slako.f90:77-394,
+ * called at DFTB/SKMatrixElements.py:217-223, 346-352).  Device arrays:
+ * pos [nat][3] (bohr), orb_atom / orb_l / orb_m [norb] (atom, l, real m of each
+ * orbital, m = -1, 0, 1 -> y, z, x), atom_type [nat] (0 = lightest element),
+ * orben [norb] (H0 diagonal), knots [npair][npts] and coefs
+ * [npair][SLAKO_NTAB][npts] (rows 0-6 S tables, 7-13 H0 tables, 14-28 dipole
+ * tables, by the reference's table index; B-spline knots and coefficients of
+ * scipy's splrep, cubic) with pair index tmax (tmax + 1) / 2 + tmin.  Splines
+ * are evaluated as splines.f90:25-94 does (interval l = INT((x - t1) / dt) + 1,
+ * clamped to the end values outside the table), pairs at r >= cutoff are zero.
+ * Elements of an unsupported input (l > 1, pair index >= npair) are NaN.
+ * H0 = NULL skips H0; the matrices are exactly symmetric. */
+enum { SLAKO_NTAB = 29 };
+int dftb_slako_matrices(const double* pos, const int* orb_atom, const int* orb_l, const int* orb_m,
+                        const int* atom_type, const double* orben, int nat, int norb, const double* knots,
+                        const double* coefs, int npair, int npts, double cutoff, double* S, double* H0, double* D,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

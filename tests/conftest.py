@@ -16,10 +16,10 @@ def pytest_configure(config):
 _GS_CACHE = {}
 
 
-def _ct_dimer(separation=30.0):
+def _ct_dimer(separation=40.0):
     """All-trans C8H10 centred at the origin and H2 (0.74 Angstrom) on the z axis
-    at `separation` bohr: two centrosymmetric fragments far beyond the
-    Slater-Koster cutoff, so every orbital lives on one fragment, and a bound
+    at `separation` bohr: two centrosymmetric fragments beyond the 30 bohr
+    Slater-Koster proximity cutoff, so every orbital lives on one fragment, and a bound
     polyene LUMO (eps < -1/separation) that makes the H2 -> C8H10 excitations
     charge-transfer states the correction shifts."""
     from dftbaby_amd.groundstate import geometry as g

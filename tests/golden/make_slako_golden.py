@@ -1,0 +1,35 @@
+"""Write tests/golden/slako_f90_golden.npz: S, H0 and the dipole matrix D from
+DFTBaby's own Fortran (DFTB/extensions/slako.f90 H0andS and DipoleMatrix, with
+splines.f90), built from the reference tree by oracle/Makefile into oracle/_ref,
+for the geometries of tests/test_slako.py::golden_geometries.  Run in a container
+that has the reference tree:  python tests/golden/make_slako_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from oracle import slako_oracle as K  # noqa: E402
+from test_slako import f90_lib, f90_matrices, golden_geometries  # noqa: E402
+
+
+def main():
+    lib = f90_lib()
+    tabs = K.load_tables()
+    data = {}
+    for name, Z, pos in golden_geometries():
+        S, H, D = f90_matrices(lib, Z, pos, *tabs)
+        data.update({name + "_Z": np.asarray(Z, dtype=np.int64), name + "_pos": np.asarray(pos), name + "_S": S,
+                     name + "_H0": H, name + "_D": D})
+    out = os.path.join(HERE, "slako_f90_golden.npz")
+    np.savez_compressed(out, **data)
+    print("wrote", out, os.path.getsize(out), "bytes")
+
+
+if __name__ == "__main__":
+    main()

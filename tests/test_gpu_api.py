@@ -4,7 +4,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import ground_state
+from conftest import ROOT, ground_state
 from oracle import tddftb_oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -113,7 +113,7 @@ def test_error_paths():
 
 @pytest.mark.parametrize("ifact", [None, 1, 4])
 def test_ct_correction(ifact):
-    """ct_correction=1 (LR_TDDFTB.py:429-437) on C8H10 + H2 30 bohr apart: the
+    """ct_correction=1 (LR_TDDFTB.py:429-437) on C8H10 + H2 40 bohr apart: the
     shifts of LongRangeCT.py:13-93 computed on the device equal the oracle's; the
     dense path (ifact None) equals the oracle's dense Casida with the same shifts;
     the Davidson (14 roots) equals the oracle's HermitianDavidson with the shifts
@@ -295,7 +295,8 @@ def test_f2py_module_dropin_against_fortran_golden():
 
 class _DenseDipoleGroundState:
     """A ground state exposing only DFTBaby's DFTB2 getters (DFTB2.py:2405-2454):
-    a dense getDipoleMatrix and no getOrbitalCenters, as DFTBaby's own object."""
+    a numpy getDipoleMatrix [norb][norb][3] and no device planes, as DFTBaby's
+    own object."""
 
     def __init__(self, gs):
         self._gs = gs
@@ -303,29 +304,34 @@ class _DenseDipoleGroundState:
                      "verbose"):
             setattr(self, name, getattr(gs, name))
         self.en = gs.en_elec
+        self.dipole_calls = 0
 
     def getEnergy(self, **opts):
         return self.en
 
+    def getDipoleMatrix(self):
+        self.dipole_calls += 1
+        return self._gs.getDipoleMatrix()
+
     def __getattr__(self, name):
-        if name.startswith("get") and name != "getOrbitalCenters":
+        if name.startswith("get") and name != "getDipoleMatrixDevice":
             return getattr(self._gs, name)
         raise AttributeError(name)
 
 
 @pytest.mark.parametrize("lc", [0, 1])
 def test_dense_dipole_matrix_ground_state(lc):
-    """LR_TDDFTB on a ground state without getOrbitalCenters takes the dense
-    dipole-matrix path (tdip = C^T D C over all MOs, LR_TDDFTB.py:574-592); its
-    transition dipoles, oscillator strengths and MO dipole matrices equal the
-    oracle's with the same dipole matrix."""
+    """LR_TDDFTB on a ground state that only has the numpy getDipoleMatrix
+    (tdip = C^T D C, LR_TDDFTB.py:574-592); its transition dipoles, oscillator
+    strengths and MO dipole matrices equal the oracle's with the same dipole
+    matrix."""
     from dftbaby_amd.LR_TDDFTB import LR_TDDFTB
     gs = ground_state("C8H10", lc, 0.05, 23)
     wrapped = _DenseDipoleGroundState(gs)
-    assert not hasattr(wrapped, "getOrbitalCenters")
+    assert not hasattr(wrapped, "getDipoleMatrixDevice")
     td = LR_TDDFTB(ground_state=wrapped)
     Omega, osc = td.getEnergies(nstates=5, diag_conv=1e-10, diag_maxiter=100)
-    assert td._tdipMO_src is None                       # the dense path ran
+    assert wrapped.dipole_calls == 1                    # the numpy dipole matrix was used
     ref = O.solve_from_ground_state(gs, nstates=5, diag_conv=1e-10, diag_maxiter=100)
     np.testing.assert_allclose(Omega, ref["Omega"], atol=1e-10)
     # exact: the oracle's dense formula applied to this solve's own C
@@ -338,6 +344,65 @@ def test_dense_dipole_matrix_ground_state(lc):
     np.testing.assert_allclose(osc, ref["osc"], rtol=1e-7 if lc == 0 else 1e-5, atol=1e-10)
     D, orbs = gs.getDipoleMatrix(), gs.getKSCoefficients()
     np.testing.assert_allclose(td.tdipMO_Z, orbs.T @ D[:, :, 2] @ orbs, atol=1e-12)
+
+
+def test_slako_kernel_against_reference_fortran_golden():
+    """dftb_slako_matrices (S, H0, dipole planes) against the reference Fortran's
+    H0andS / DipoleMatrix outputs (tests/golden/slako_f90_golden.npz): displaced
+    C8H10, the same rotated, molecules 10-35 bohr apart (clamped table ends,
+    proximity cutoff)."""
+    from dftbaby_amd.groundstate import slako as SK
+    gd = np.load(os.path.join(ROOT, "tests", "golden", "slako_f90_golden.npz"))
+    for name in ("c8h10", "c8h10_rotated", "far_pairs"):
+        Z, pos = gd[name + "_Z"], gd[name + "_pos"]
+        S, H, D = SK.matrices_device(Z, pos, "cuda")
+        np.testing.assert_allclose(S.cpu().numpy(), gd[name + "_S"], rtol=0, atol=1e-15, err_msg=name)
+        np.testing.assert_allclose(H.cpu().numpy(), gd[name + "_H0"], rtol=0, atol=1e-15, err_msg=name)
+        Dh = np.moveaxis(D.cpu().numpy(), 0, 2)
+        np.testing.assert_allclose(Dh, gd[name + "_D"], rtol=4e-15, atol=1e-15, err_msg=name)
+
+
+def test_slako_kernel_flake_against_oracle_and_host():
+    """The kernel on the C604H68 flake (Nat 672, Norb 2484, pairs beyond the
+    10 bohr table end and the 30 bohr cutoff) against the producer's host
+    assembly (exactly symmetric, equal to a few ulps: FMA contraction), and the
+    first 24 atoms against the oracle's scalar restatement."""
+    from dftbaby_amd.groundstate import geometry as g
+    from dftbaby_amd.groundstate import slako as SK
+    from oracle import slako_oracle as K
+    fl = g.flake_c604h68()
+    Z = [a for a, _ in fl]
+    pos = np.array([x for _, x in fl])
+    S, H, D = SK.matrices_device(Z, pos, "cuda")
+    S, H, D = S.cpu().numpy(), H.cpu().numpy(), np.moveaxis(D.cpu().numpy(), 0, 2)
+    assert np.array_equal(S, S.T) and np.array_equal(H, H.T)
+    assert all(np.array_equal(D[:, :, c], D[:, :, c].T) for c in range(3))
+    Sh, Hh, Dh = SK.matrices_host(Z, pos)
+    np.testing.assert_allclose(S, Sh, rtol=8e-15, atol=1e-15)
+    np.testing.assert_allclose(H, Hh, rtol=8e-15, atol=1e-15)
+    np.testing.assert_allclose(D, Dh, rtol=8e-15, atol=1e-15)
+    # oracle on a corner of the flake: the first 24 atoms (their pairs with every
+    # other atom are covered by the host comparison above)
+    sub = list(range(24))
+    So, Ho = K.h0_and_s([Z[i] for i in sub], pos[sub])
+    Do = K.dipole_matrix([Z[i] for i in sub], pos[sub], So)
+    n = len(So)
+    np.testing.assert_allclose(S[:n, :n], So, rtol=8e-15, atol=1e-15)
+    np.testing.assert_allclose(H[:n, :n], Ho, rtol=8e-15, atol=1e-15)
+    np.testing.assert_allclose(D[:n, :n], Do, rtol=8e-15, atol=1e-15)
+
+
+def test_bundled_producer_dipoles_from_device_planes():
+    """The bundled producer hands LR_TDDFTB the kernel's dipole planes; the
+    transition dipoles equal the oracle's formula with the host dipole matrix."""
+    from dftbaby_amd.LR_TDDFTB import LR_TDDFTB
+    gs = ground_state("C8H10", 1, 0.05, 23)
+    td = LR_TDDFTB(ground_state=gs)
+    td.getEnergies(nstates=5, diag_conv=1e-10, diag_maxiter=100)
+    own = O.transition_dipoles(td.Cij, gs.getKSCoefficients(), gs.getDipoleMatrix(), td.active_occupied_orbs,
+                               td.active_virtual_orbs)
+    for c in range(3):
+        np.testing.assert_allclose((td.tdipX, td.tdipY, td.tdipZ)[c], own[c], rtol=0, atol=1e-12)
 
 
 @pytest.mark.parametrize("lc", [0, 1])

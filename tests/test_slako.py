@@ -1,0 +1,244 @@
+"""Pin the Slater-Koster oracle (oracle/slako_oracle.py) and check the producer's
+host assembly against it (CPU).
+
+* Against DFTBaby's own Fortran (DFTB/extensions/slako.f90 H0andS and
+  DipoleMatrix, with splines.f90's splev_uniform) compiled by oracle/Makefile into
+  oracle/_ref, when that build exists, and against the golden vectors it produced
+  (tests/golden/slako_f90_golden.npz, written by tests/golden/make_slako_golden.py):
+  a C8H10 polyene with displaced atoms, the same rotated, and a set of molecules
+  10-35 bohr apart (clamped table ends, the 30 bohr proximity cutoff).
+* Against what the mathematics fixes: the interval rule of splev_uniform against
+  an exact evaluation of the extrapolated cubic piece; S and H0 invariant under a
+  rotation of the molecule, D covariant off the atoms' own blocks; the tensor
+  rules against the generated-rule expansions for a sample of (l1, m1, mM, l2, m2).
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+from scipy.interpolate import PPoly
+
+from conftest import ROOT
+from oracle import slako_oracle as K
+
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libslako_ref.so")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "slako_f90_golden.npz")
+
+
+def f90_lib():
+    if not os.path.exists(REF_SO):
+        pytest.skip("oracle/_ref not built (needs the reference tree and amdflang)")
+    return ctypes.CDLL(REF_SO)
+
+
+def _f90_tables(tabs):
+    """The arrays combine_slako_tables_f90 (SKIntegrals.py:1078-1188) passes to the
+    Fortran: atom types H = 0, C = 1, pair index j(j+1)/2 + i."""
+    types = {1: 0, 6: 1}
+    npair = 3
+    nsk = 1 + max(max(tabs[p][k].keys()) for p in tabs for k in ("S", "H"))
+    nskd = 1 + max(max(tabs[p]["D"].keys()) for p in tabs)
+    npts = len(next(iter(tabs[(1, 1)]["S"].values()))[0])
+    arr = {n: np.zeros((npair, nsk if n[0] in "SH" else nskd, npts), order="F")
+           for n in ("S_knots", "S_coefs", "H_knots", "H_coefs", "D_knots", "D_coefs")}
+    filled = np.zeros((npair, nsk), dtype=np.int32, order="F")
+    filled_d = np.zeros((npair, nskd), dtype=np.int32, order="F")
+    for (z1, z2), entry in tabs.items():
+        i, j = types[z1], types[z2]
+        ip = j * (j + 1) // 2 + i
+        for kind in ("S", "H", "D"):
+            for idx, (t, c, k) in entry[kind].items():
+                arr[kind + "_knots"][ip, idx, :] = t
+                arr[kind + "_coefs"][ip, idx, :] = c
+                (filled_d if kind == "D" else filled)[ip, idx] = 1
+    return types, filled, filled_d, arr, (npair, nsk, nskd, npts)
+
+
+def f90_matrices(lib, Z, pos, tabs, orben, cutoff=30.0):
+    """S, H0 and D from the reference Fortran (D as DipoleMatrix returns it, rolled
+    to [mu][nu][xyz] like SKMatrixElements.py:355)."""
+    types, filled, filled_d, arr, (npair, nsk, nskd, npts) = _f90_tables(tabs)
+    Z = [int(z) for z in Z]
+    nat = len(Z)
+    ind, typ, ls, ms, en = [], [], [], [], []
+    for A, z in enumerate(Z):
+        for o, (l, m) in enumerate(K.VALORBS[z]):
+            ind.append(A)
+            typ.append(types[z])
+            ls.append(l)
+            ms.append(m)
+            en.append(orben[z][o])
+    norb = len(ind)
+    I = lambda a: np.asfortranarray(np.asarray(a, dtype=np.int32))
+    ind, typ, ls, ms = I(ind), I(typ), I(ls), I(ms)
+    en = np.asfortranarray(np.asarray(en, dtype=np.float64))
+    P = np.asfortranarray(np.asarray(pos, dtype=np.float64).T)          # pos(3, Nat)
+    r, x, y, z = (np.zeros((nat, nat), order="F") for _ in range(4))
+    ip = lambda v: ctypes.byref(ctypes.c_int(v))
+    dp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib._QMslakoPdirectional_cosines(dp(P), dp(P), ip(nat), ip(nat), dp(r), dp(x), dp(y), dp(z))
+    mprox = np.zeros((nat, nat), dtype=np.int32, order="F")
+    for i in range(nat):
+        mprox[i, i] = 1
+        for j in range(i + 1, nat):
+            if r[j, i] < cutoff:
+                mprox[i, j] = mprox[j, i] = 1
+    S = np.zeros((norb, norb), order="F")
+    H = np.zeros((norb, norb), order="F")
+    lib._QMslakoPh0ands(ip(norb), ip(nat), dp(ind), dp(typ), dp(ls), dp(ms), dp(ind), dp(typ), dp(ls), dp(ms),
+                        dp(r), dp(x), dp(y), dp(z), dp(mprox), dp(en), ip(3), dp(filled),
+                        dp(arr["S_knots"]), dp(arr["S_coefs"]), dp(arr["H_knots"]), dp(arr["H_coefs"]),
+                        ip(npair), ip(nsk), ip(npts), dp(S), dp(H))
+    D = np.zeros((3, norb, norb), order="F")
+    lib._QMslakoPdipolematrix(ip(norb), ip(nat), dp(ind), dp(typ), dp(ls), dp(ms), dp(ind), dp(typ), dp(ls),
+                              dp(ms), dp(r), dp(x), dp(y), dp(z), dp(P), dp(mprox), dp(S), ip(3), dp(filled_d),
+                              dp(arr["D_knots"]), dp(arr["D_coefs"]), ip(npair), ip(nskd), ip(npts), dp(D))
+    return np.array(S), np.array(H), np.ascontiguousarray(np.moveaxis(D, 0, 2))
+
+
+def golden_geometries():
+    """(name, Z, pos) of the golden cases."""
+    from dftbaby_amd.groundstate import geometry as g
+    out = []
+    mol = g.displace(g.polyene(8), 0.05, 21)
+    Z = [a for a, _ in mol]
+    pos = np.array([x for _, x in mol])
+    out.append(("c8h10", Z, pos))
+    th = 0.7
+    R = np.array([[np.cos(th), 0.0, np.sin(th)], [0.0, 1.0, 0.0], [-np.sin(th), 0.0, np.cos(th)]])
+    R = R @ np.array([[1.0, 0.0, 0.0], [0.0, np.cos(0.4), -np.sin(0.4)], [0.0, np.sin(0.4), np.cos(0.4)]])
+    out.append(("c8h10_rotated", Z, pos @ R.T))
+    eth = g.displace(g.ethylene(), 0.03, 4)
+    Ze = [a for a, _ in eth]
+    pe = np.array([x for _, x in eth])
+    far = [(Ze, pe), (Ze, pe + [0.0, 3.0, 14.0]), ([1, 1], np.array([[0.0, 20.0, 19.0], [0.0, 20.0, 20.4]])),
+           ([1], np.array([[0.0, -36.0, 0.0]])), ([6, 1], np.array([[25.0, 0.0, 0.0], [27.06, 0.0, 0.0]]))]
+    out.append(("far_pairs", sum((list(z) for z, _ in far), []), np.vstack([p for _, p in far])))
+    return out
+
+
+# ----------------------------------------------------------------------------- pins
+def test_oracle_against_reference_fortran():
+    lib = f90_lib()
+    tabs = K.load_tables()
+    for name, Z, pos in golden_geometries():
+        S, H = K.h0_and_s(Z, pos, tabs)
+        D = K.dipole_matrix(Z, pos, S, tabs)
+        Sf, Hf, Df = f90_matrices(lib, Z, pos, *tabs)
+        np.testing.assert_allclose(S, Sf, rtol=0, atol=1e-15, err_msg=name)
+        np.testing.assert_allclose(H, Hf, rtol=0, atol=1e-15, err_msg=name)
+        np.testing.assert_allclose(D, Df, rtol=0, atol=1e-14, err_msg=name)
+
+
+def test_oracle_against_golden_vectors():
+    gd = np.load(GOLDEN)
+    tabs = K.load_tables()
+    for name in ("c8h10", "c8h10_rotated", "far_pairs"):
+        Z, pos = gd[name + "_Z"], gd[name + "_pos"]
+        S, H = K.h0_and_s(Z, pos, tabs)
+        D = K.dipole_matrix(Z, pos, S, tabs)
+        np.testing.assert_allclose(S, gd[name + "_S"], rtol=0, atol=1e-15, err_msg=name)
+        np.testing.assert_allclose(H, gd[name + "_H0"], rtol=0, atol=1e-15, err_msg=name)
+        np.testing.assert_allclose(D, gd[name + "_D"], rtol=0, atol=1e-14, err_msg=name)
+
+
+def test_splev_uniform_interval_rule():
+    """Inside the table the evaluation is the spline's cubic piece two intervals to
+    the left, extrapolated (PPoly of the same tck evaluated on that piece); at and
+    beyond the ends it is clamped; a dropped '+1' or a correct interval lookup
+    fails the comparison."""
+    tabs, _ = K.load_tables()
+    tck = tabs[(6, 6)]["S"][6]
+    t, c, k = tck
+    pp = PPoly.from_spline(tck)
+    grid = t[3:-3]                           # d[0], d[2], d[3], ..., d[-1]
+    h = t[5] - t[4]
+    rng = np.random.default_rng(0)
+    off = []
+    for x in rng.uniform(0.5, 9.8, 40):
+        j = int((x - t[0]) / h)              # x in [d_j, d_j+1)
+        piece_left = (j - 2) * h             # left end of [d_{j-2}, d_{j-1})
+        ip = np.searchsorted(pp.x, piece_left + 0.5 * h, side="right") - 1
+        coef = pp.c[:, ip]
+        dx = x - pp.x[ip]
+        want = ((coef[0] * dx + coef[1]) * dx + coef[2]) * dx + coef[3]
+        got = K.splev_uniform(tck, x)
+        assert abs(got - want) < 1e-13 * max(1.0, abs(want)), (x, got, want)
+        off.append(abs(got - float(pp(x))))
+    # the shifted interval is visible: far above rounding against the spline itself
+    assert max(off) > 1e-8
+    assert K.splev_uniform(tck, 12.0) == pytest.approx(float(pp(grid[-1])), abs=1e-15)
+    assert K.splev_uniform(tck, 25.0) == K.splev_uniform(tck, grid[-1])
+
+
+def test_rotation_invariance_and_dipole_covariance():
+    """S, H0 are scalars under a rotation Q of the molecule once the p orbitals are
+    rotated along (S' = U S U^T with U the block rotation of (p_x, p_y, p_z)); D
+    rotates as a vector in both its orbital indices and its component."""
+    from dftbaby_amd.groundstate import geometry as g
+    mol = g.displace(g.polyene(8), 0.05, 3)
+    Z = [a for a, _ in mol]
+    pos = np.array([x for _, x in mol])
+    tabs = K.load_tables()
+    th = 1.1
+    Q = np.array([[np.cos(th), -np.sin(th), 0.0], [np.sin(th), np.cos(th), 0.0], [0.0, 0.0, 1.0]])
+    Q = Q @ np.array([[1.0, 0.0, 0.0], [0.0, np.cos(0.3), -np.sin(0.3)], [0.0, np.sin(0.3), np.cos(0.3)]])
+    S, H = K.h0_and_s(Z, pos, tabs)
+    D = K.dipole_matrix(Z, pos, S, tabs)
+    S2, H2 = K.h0_and_s(Z, pos @ Q.T, tabs)
+    D2 = K.dipole_matrix(Z, pos @ Q.T, S2, tabs)
+    # orbital rotation: (y, z, x) ordering of the real p orbitals
+    perm = [1, 2, 0]
+    Qp = Q[np.ix_(perm, perm)]
+    n = len(S)
+    U = np.zeros((n, n))
+    o = 0
+    for z in Z:
+        if z == 1:
+            U[o, o] = 1.0
+            o += 1
+        else:
+            U[o, o] = 1.0
+            U[o + 1:o + 4, o + 1:o + 4] = Qp
+            o += 4
+    np.testing.assert_allclose(U @ S @ U.T, S2, atol=1e-12)
+    np.testing.assert_allclose(U @ H @ U.T, H2, atol=1e-12)
+    # off-site blocks rotate; same-atom blocks are evaluated on the fixed axis
+    # (0, 0, 1) of slako.f90:339-345 (r = 0), so they only move with their atom:
+    # D2 - D = (R2_A - R_A) delta_mu,nu
+    amap = np.repeat(np.arange(len(Z)), [1 if z == 1 else 4 for z in Z])
+    same = amap[:, None] == amap[None, :]
+    Drot = np.einsum("cd,mnd->mnc", Q, np.einsum("am,mnc,bn->abc", U, D, U))
+    np.testing.assert_allclose(Drot[~same], D2[~same], atol=1e-10)
+    shift = (pos @ Q.T - pos)[amap]
+    want = D + np.eye(n)[:, :, None] * shift[:, None, :]
+    np.testing.assert_allclose(D2[same], want[same], atol=1e-12)
+
+
+def test_tensor_rules_against_generated_expansions():
+    """A few of the generated rules (slako_transformations_dipole_lmax2.f90 /
+    slako_transformations_lmax2.f90 cases for s and p), written out here as
+    polynomials in the direction cosines, against the tensor forms."""
+    rng = np.random.default_rng(2)
+    for _ in range(5):
+        e = rng.standard_normal(3)
+        e /= np.linalg.norm(e)
+        x, y, z = e
+        T = rng.standard_normal(7)
+        Dt = rng.standard_normal(15)
+        # <p_y|p_y> = (x^2 + z^2) pp_pi + y^2 pp_sigma; <p_y|p_z> = y z (pp_sigma - pp_pi)
+        assert K._sk(1, -1, 1, -1, e, T) == pytest.approx((x * x + z * z) * T[5] + y * y * T[6], abs=1e-14)
+        assert K._sk(1, -1, 1, 0, e, T) == pytest.approx(y * z * (T[6] - T[5]), abs=1e-14)
+        assert K._sk(1, 1, 0, 0, e, T) == pytest.approx(x * T[4], abs=1e-14)
+        # <s|y|p_y>, <p_y|y|p_z>, <p_y|z|p_x>, <p_z|z|p_z>
+        assert K._sk_dipole(0, 0, 1, 1, -1, e, Dt) == pytest.approx((x * x + z * z) * Dt[1] + y * y * Dt[4],
+                                                                    abs=1e-14)
+        assert K._sk_dipole(1, -1, 1, 1, 0, e, Dt) == pytest.approx(
+            z * ((x * x + z * z) * Dt[7] + y * y * (-2 * Dt[8] + Dt[14])), abs=1e-13)
+        assert K._sk_dipole(1, -1, 2, 1, 1, e, Dt) == pytest.approx(x * y * z * (-Dt[7] - 2 * Dt[8] + Dt[14]),
+                                                                   abs=1e-13)
+        assert K._sk_dipole(1, -1, 2, 1, -1, e, Dt) == pytest.approx(
+            z * (-(y * y * Dt[7]) + (x * x - y * y + z * z) * Dt[8] + y * y * Dt[14]), abs=1e-13)
+        assert K._sk_dipole(1, -1, 1, 0, 0, e, Dt) == pytest.approx((x * x + z * z) * Dt[6] + y * y * Dt[13],
+                                                                   abs=1e-14)
