@@ -14,104 +14,10 @@
 #include "common.h"
 #include "dftbaby_amd.h"
 
+#include "slako_device.h"
+
 namespace {
-
-constexpr int kDeg = 3;   // cubic B-splines (splrep's default degree)
-
-// B-spline basis and knot interval for one distance, as splev_uniform/fpbspl.
-struct Basis {
-  double h[kDeg + 1];
-  int c0;   // first coefficient index (0-based)
-};
-
-__device__ Basis spline_basis(const double* __restrict__ t, int n, double x) {
-  const int k1 = kDeg + 1, nk1 = n - k1;
-  const double tb = t[k1 - 1], te = t[nk1];
-  double arg;
-  int l;   // 1-based knot interval, as in splines.f90
-  if (x <= tb) {
-    arg = tb;
-    l = k1;
-  } else if (x >= te) {
-    arg = te;
-    l = nk1;
-  } else {
-    arg = x;
-    const double dt = t[k1 + 1] - t[k1];
-    l = (int)((x - t[0]) / dt) + 1;
-  }
-  // splines.f90 moves l < k to k+1; l == k (x within three grid steps of the
-  // first point) would read before the coefficient array there.  Distinct atoms
-  // are never that close; clamp so the read stays in bounds.
-  if (l < k1) l = k1;
-  if (l > nk1) l = nk1;
-  Basis b;
-  b.h[0] = 1.0;
-#pragma unroll
-  for (int j = 1; j <= kDeg; ++j) {
-    double hh[kDeg];
-#pragma unroll
-    for (int i = 0; i < j; ++i) hh[i] = b.h[i];
-    b.h[0] = 0.0;
-#pragma unroll
-    for (int i = 1; i <= j; ++i) {
-      const int li = l + i, lj = li - j;
-      const double f = hh[i - 1] / (t[li - 1] - t[lj - 1]);
-      b.h[i - 1] = b.h[i - 1] + f * (t[li - 1] - arg);
-      b.h[i] = f * (arg - t[lj - 1]);
-    }
-  }
-  b.c0 = l - k1;
-  return b;
-}
-
-__device__ __forceinline__ double spline_value(const double* __restrict__ c, const Basis& b) {
-  double s = 0.0;
-#pragma unroll
-  for (int j = 0; j <= kDeg; ++j) s = s + c[b.c0 + j] * b.h[j];
-  return s;
-}
-
-// real p orbital m = -1, 0, 1 -> Cartesian y, z, x
-__device__ __forceinline__ int pidx(int m) { return m == -1 ? 1 : (m == 0 ? 2 : 0); }
-
-__device__ __forceinline__ double sum_other_sq(const double* e, int i) {
-  const double x2 = e[0] * e[0], y2 = e[1] * e[1], z2 = e[2] * e[2];
-  return i == 0 ? y2 + z2 : (i == 1 ? x2 + z2 : x2 + y2);
-}
-
-// <first|second> from the table values T[0..6] (ss-sigma 0, sp-sigma 2,
-// ps-sigma 4, pp-pi 5, pp-sigma 6), e: unit vector first -> second.
-__device__ double sk_rule(int l1, int m1, int l2, int m2, const double* e, const double* T) {
-  if (l1 == 0 && l2 == 0) return T[0];
-  if (l1 == 0) return e[pidx(m2)] * T[2];
-  if (l2 == 0) return e[pidx(m1)] * T[4];
-  const int i = pidx(m1), j = pidx(m2);
-  if (i == j) return sum_other_sq(e, i) * T[5] + e[i] * e[i] * T[6];
-  return e[i] * e[j] * (T[6] - T[5]);
-}
-
-// <first|r_M|second> (without the origin shift) from the dipole tables D[0..14]
-// (indices 1, 3, 4, 6, 7, 8, 13, 14 carry data for s and p).
-__device__ double sk_rule_dipole(int l1, int m1, int M, int l2, int m2, const double* e, const double* D) {
-  if (l1 == 0 && l2 == 0) return e[M] * D[3];
-  if (l1 == 0) {
-    const int j = pidx(m2);
-    if (M == j) return sum_other_sq(e, j) * D[1] + e[j] * e[j] * D[4];
-    return e[M] * e[j] * (D[4] - D[1]);
-  }
-  if (l2 == 0) {
-    const int i = pidx(m1);
-    if (M == i) return sum_other_sq(e, i) * D[6] + e[i] * e[i] * D[13];
-    return e[i] * e[M] * (D[13] - D[6]);
-  }
-  const int i = pidx(m1), j = pidx(m2);
-  double v = (D[14] - D[7] - 2.0 * D[8]) * e[i] * e[M] * e[j];
-  if (i == M) v += D[7] * e[j];
-  if (M == j) v += D[8] * e[i];
-  if (i == j) v += D[8] * e[M];
-  return v;
-}
+using namespace slako_dev;
 
 __global__ void __launch_bounds__(256) slako_kernel(const double* __restrict__ pos, const int* __restrict__ orb_atom,
                                                     const int* __restrict__ orb_l, const int* __restrict__ orb_m,
@@ -208,7 +114,7 @@ extern "C" int dftb_slako_matrices(const double* pos, const int* orb_atom, const
                                    const int* atom_type, const double* orben, int nat, int norb,
                                    const double* knots, const double* coefs, int npair, int npts, double cutoff,
                                    double* S, double* H0, double* D, void* stream) {
-  if (nat <= 0 || norb <= 0 || npair <= 0 || npts < 2 * (kDeg + 1) + 2) return DFTB_ERR_ARG;
+  if (nat <= 0 || norb <= 0 || npair <= 0 || npts < 2 * (slako_dev::kDeg + 1) + 2) return DFTB_ERR_ARG;
   if (!pos || !orb_atom || !orb_l || !orb_m || !atom_type || !orben || !knots || !coefs || !S) return DFTB_ERR_ARG;
   if (!(cutoff > 0.0)) return DFTB_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;

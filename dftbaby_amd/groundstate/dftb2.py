@@ -212,9 +212,19 @@ class DFTB2:
         self.dq = dq.cpu().numpy()
         self.scc_iterations = it + 1
         _, self.f = fermi_occupation_T0(self.orbe, self.Nelec_val)
-        # electronic energy (band structure + second order + lc exchange), without repulsion
-        E = float((P * H0t).sum() + 0.5 * dq @ G @ dq)
+        # electronic energy (DFTB2.py:2241-2254): band structure + second order +
+        # long-range exchange of the difference density (DFTB2.py:1267-1285); no
+        # repulsive energy
+        self.E_bs = float((P * H0t).sum())
+        self.E_coulomb = float(0.5 * dq @ G @ dq)
+        E = self.E_bs + self.E_coulomb
+        self.E_HF_x = 0.0
+        if self.long_range_correction == 1:
+            dP = P - P0
+            self.E_HF_x = float(-((St @ dP @ St) * dP * Glr_ao).sum() - ((St @ dP) * (dP @ St) * Glr_ao).sum()) / 8.0
+            E += self.E_HF_x
         self.en_elec = E
+        self.E_elec = E
         return E
 
     # -- getters used by the excited-state path (DFTB/DFTB2.py:2405-2454) --------

@@ -57,6 +57,18 @@ class Tables:
                     coefs[ip, base + int(key[2])] = c
         self.knots, self.coefs = knots, coefs
         self.npts = knots.shape[1]
+        # first-derivative coefficients of the S and H0 tables (FITPACK splder's
+        # recurrence, a degree-2 spline on the knots t[1..npts-2])
+        n = self.npts
+        nk1 = n - 4
+        self.dcoefs = np.zeros((self.npair, 14, n))
+        for ip in range(self.npair):
+            t = knots[ip]
+            fac = t[np.arange(nk1 - 1) + 4] - t[np.arange(nk1 - 1) + 1]
+            for row in range(14):
+                c = coefs[ip, row]
+                diff = c[1:nk1] - c[:nk1 - 1]
+                self.dcoefs[ip, row, :nk1 - 1] = np.where(fac > 0.0, 3.0 * diff / np.where(fac > 0.0, fac, 1.0), 0.0)
         self.orbital_energies = {1: [float(d["h_energy_1s"])],
                                  6: [float(d["c_energy_2s"])] + 3 * [float(d["c_energy_2p"])]}
         self._dev = {}
@@ -66,7 +78,8 @@ class Tables:
         key = str(dev)
         if key not in self._dev:
             self._dev[key] = (torch.as_tensor(self.knots, device=dev).contiguous(),
-                              torch.as_tensor(self.coefs, device=dev).contiguous())
+                              torch.as_tensor(self.coefs, device=dev).contiguous(),
+                              torch.as_tensor(self.dcoefs, device=dev).contiguous())
         return self._dev[key]
 
 
@@ -109,7 +122,7 @@ def matrices_device(Z, pos, dev, distance_cutoff=30.0, dipoles=True):
     P = t(np.asarray(pos, dtype=np.float64).reshape(nat, 3), torch.float64)
     oa, ol, om, ty = (t(a, torch.int32) for a in (atom, ls, ms, types))
     e = t(en, torch.float64)
-    knots, coefs = T.device_arrays(dev)
+    knots, coefs, _ = T.device_arrays(dev)
     S = torch.empty((norb, norb), dtype=torch.float64, device=dev)
     H = torch.empty((norb, norb), dtype=torch.float64, device=dev)
     D = torch.empty((3, norb, norb), dtype=torch.float64, device=dev) if dipoles else None

@@ -344,6 +344,25 @@ int dftb_slako_matrices(const double* pos, const int* orb_atom, const int* orb_l
                         const double* coefs, int npair, int npts, double cutoff, double* S, double* H0, double* D,
                         void* stream);
 
+/* Nuclear-derivative contractions of the LC-TD-DFTB gradient: for every atom C,
+ *   grad[3C + c] = <AS, dS/dR_Cc> + <AH, dH0/dR_Cc> + <BG, dgamma_AO/dR_Cc>
+ *                + <BL, dgamma_lr_AO/dR_Cc>        (<X, Y> = sum_mn X_mn Y_mn)
+ * with the derivative tensors of slako.f90 gradients_H0andS (402-569) and
+ * GammaApproximation.gamma_AOwise (388-450), as ExcGradients.gradient_lc
+ * (ExcGradients.py:286-711) contracts them, evaluated per atom pair and never
+ * stored.  AS/AH/BG/BL: [norb][norb] device matrices (NULL = term absent);
+ * atom_off [nat+1] orbital offsets; sigma [nat] Gaussian widths 1/(sqrt(pi) U);
+ * rlr the long-range radius; knots / coefs as dftb_slako_matrices; dcoefs
+ * [npair][14][npts] the first-derivative B-spline coefficients of the S (rows
+ * 0-6) and H0 (7-13) tables (splder's recurrence, on the knots t[1..npts-2]).
+ * grad [3 nat] is written (not accumulated).  No reference counterpart as one
+ * function: it replaces the tensordot contractions of ExcGradients.py:477-496,
+ * 694-711 over the [3 Nat][Norb][Norb] tensors. */
+int dftb_grad_pairs(const double* pos, const int* atom_type, const int* atom_off, const int* orb_l, const int* orb_m,
+                    const double* sigma, double rlr, const double* knots, const double* coefs, const double* dcoefs,
+                    int npair, int npts, double cutoff, const double* AS, const double* AH, const double* BG,
+                    const double* BL, int nat, int norb, double* grad, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

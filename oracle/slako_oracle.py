@@ -97,6 +97,46 @@ def fpbspl(t, k, x, l):
     return h
 
 
+def _interval(t, k, x):
+    """the knot interval l (1-based) with t(l) <= x < t(l+1), x inside [t(k+1), t(n-k)]."""
+    n = len(t)
+    l = k + 1
+    while x >= t[l] and l != n - k - 1:
+        l += 1
+    return l
+
+
+def splev_exact(tck, x):
+    """The spline itself (correct knot interval; clamped outside the table), for
+    checks by finite differences."""
+    t, c, k = tck
+    n = len(t)
+    arg = min(max(x, t[k]), t[n - k - 1])
+    l = _interval(t, k, arg)
+    h = fpbspl(t, k, arg, l)
+    return sum(c[l - k - 1 + j] * h[j] for j in range(k + 1))
+
+
+def splder1(tck, x):
+    """First derivative as FITPACK's splder (splines.f90:97-115 -> splder.f:
+    derivative coefficients by de Boor's recurrence, x clamped to [t(k+1), t(n-k)],
+    knot interval found by search)."""
+    t, c, k = tck
+    n = len(t)
+    nk1 = n - k - 1
+    w = [c[i + 1] - c[i] for i in range(nk1 - 1)]
+    for i in range(nk1 - 1):
+        fac = t[i + k + 1] - t[i + 1]
+        w[i] = k * w[i] / fac if fac > 0.0 else 0.0
+    # degree k-1 spline on the knots t[1:n-1]
+    t1 = t[1:n - 1]
+    arg = min(max(x, t[k]), t[nk1])
+    k2 = k - 1
+    l = _interval(t1, k2, arg)
+    h = fpbspl(t1, k2, arg, l)
+    return sum(w[l - k2 - 1 + j] * h[j] for j in range(k2 + 1))
+
+
 def splev_uniform(tck, x):
     """splines.f90:25-94 for one point x (see the module doc for the interval rule)."""
     t, c, k = tck

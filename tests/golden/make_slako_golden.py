@@ -1,7 +1,8 @@
 """Write tests/golden/slako_f90_golden.npz: S, H0 and the dipole matrix D from
 DFTBaby's own Fortran (DFTB/extensions/slako.f90 H0andS and DipoleMatrix, with
 splines.f90), built from the reference tree by oracle/Makefile into oracle/_ref,
-for the geometries of tests/test_slako.py::golden_geometries.  Run in a container
+for the geometries of tests/test_slako.py::golden_geometries, and dS/dR, dH0/dR
+(gradients_H0andS) for those of gradient_geometries.  Run in a container
 that has the reference tree:  python tests/golden/make_slako_golden.py
 """
 import os
@@ -15,7 +16,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from oracle import slako_oracle as K  # noqa: E402
-from test_slako import f90_lib, f90_matrices, golden_geometries  # noqa: E402
+from test_slako import f90_gradients, f90_lib, f90_matrices, golden_geometries, gradient_geometries  # noqa: E402
 
 
 def main():
@@ -26,6 +27,10 @@ def main():
         S, H, D = f90_matrices(lib, Z, pos, *tabs)
         data.update({name + "_Z": np.asarray(Z, dtype=np.int64), name + "_pos": np.asarray(pos), name + "_S": S,
                      name + "_H0": H, name + "_D": D})
+    for name, Z, pos in gradient_geometries():
+        gS, gH = f90_gradients(lib, Z, pos, tabs[0])
+        data.update({name + "_Z": np.asarray(Z, dtype=np.int64), name + "_pos": np.asarray(pos),
+                     name + "_gradS": gS, name + "_gradH0": gH})
     out = os.path.join(HERE, "slako_f90_golden.npz")
     np.savez_compressed(out, **data)
     print("wrote", out, os.path.getsize(out), "bytes")

@@ -97,6 +97,51 @@ def f90_matrices(lib, Z, pos, tabs, orben, cutoff=30.0):
     return np.array(S), np.array(H), np.ascontiguousarray(np.moveaxis(D, 0, 2))
 
 
+def f90_gradients(lib, Z, pos, tabs, cutoff=30.0):
+    """dS/dR and dH0/dR [3 Nat][Norb][Norb] from the reference Fortran
+    (slako.f90 gradients_H0andS)."""
+    types, filled, _, arr, (npair, nsk, _, npts) = _f90_tables(tabs)
+    Z = [int(z) for z in Z]
+    nat = len(Z)
+    ind, typ, ls, ms = [], [], [], []
+    for A, z in enumerate(Z):
+        for (l, m) in K.VALORBS[z]:
+            ind.append(A)
+            typ.append(types[z])
+            ls.append(l)
+            ms.append(m)
+    norb = len(ind)
+    I = lambda a: np.asfortranarray(np.asarray(a, dtype=np.int32))
+    ind, typ, ls, ms = I(ind), I(typ), I(ls), I(ms)
+    P = np.asfortranarray(np.asarray(pos, dtype=np.float64).T)
+    r, x, y, z = (np.zeros((nat, nat), order="F") for _ in range(4))
+    ip = lambda v: ctypes.byref(ctypes.c_int(v))
+    dp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib._QMslakoPdirectional_cosines(dp(P), dp(P), ip(nat), ip(nat), dp(r), dp(x), dp(y), dp(z))
+    mprox = np.zeros((nat, nat), dtype=np.int32, order="F")
+    for i in range(nat):
+        mprox[i, i] = 1
+        for j in range(i + 1, nat):
+            if r[j, i] < cutoff:
+                mprox[i, j] = mprox[j, i] = 1
+    gS = np.zeros((3 * nat, norb, norb), order="F")
+    gH = np.zeros((3 * nat, norb, norb), order="F")
+    lib._QMslakoPgradients_h0ands(ip(norb), ip(nat), dp(ind), dp(typ), dp(ls), dp(ms), dp(r), dp(x), dp(y), dp(z),
+                                  dp(mprox), ip(3), dp(filled), dp(arr["S_knots"]), dp(arr["S_coefs"]),
+                                  dp(arr["H_knots"]), dp(arr["H_coefs"]), ip(npair), ip(nsk), ip(npts), dp(gS), dp(gH))
+    return np.array(gS), np.array(gH)
+
+
+def gradient_geometries():
+    """(name, Z, pos) of the gradient golden cases: displaced ethylene, and two
+    ethylenes 14 bohr apart (table ends, clamped derivatives)."""
+    from dftbaby_amd.groundstate import geometry as g
+    eth = g.displace(g.ethylene(), 0.05, 7)
+    Ze = [a for a, _ in eth]
+    pe = np.array([x for _, x in eth])
+    return [("ethylene", Ze, pe), ("ethylene_pair", Ze + Ze, np.vstack([pe, pe + [0.0, 3.0, 14.0]]))]
+
+
 def golden_geometries():
     """(name, Z, pos) of the golden cases."""
     from dftbaby_amd.groundstate import geometry as g
