@@ -28,8 +28,10 @@ What differs is how they are formed (identical results to rounding):
   of gamma_lr on orbital pairs - no q_oo / q_vv.
 
 Only lc = 1 (``gradient_lc``) is built; ``gradient_nolc`` raises
-NotImplementedError.  The bundled ground state has no repulsive potential, so
-gradVrep is zero (``gradient_Vrep``, ``:181-210``, needs the repulsive tables).
+NotImplementedError.  gradVrep (``gradient_Vrep``, ``:181-210``) comes from the
+``dftb_grad_vrep`` kernel on the repulsive-potential splines (their derivative
+is clamped below the first table point, where scipy would extrapolate; atoms
+that close do not occur).
 A restricted active space is handled as the reference does: the amplitudes are
 padded with zeros to the full space (``:236-263``).
 """
@@ -133,6 +135,16 @@ class Gradients:
                   _p(self.k_sigma), ctypes.c_double(float(gs.long_range_radius)), _p(self.k_knots),
                   _p(self.k_coefs), _p(self.k_dcoefs), self.k_T.npair, self.k_T.npts, ctypes.c_double(cutoff),
                   _p(self.AS), _p(self.AH), _p(self.BG), _p(self.BL), nat, n, _p(out), _stream())
+        return out.cpu().numpy()
+
+    def _grad_vrep(self):
+        """gradient_Vrep (ExcGradients.py:181-210) by dftb_grad_vrep."""
+        nat = len(self.Z)
+        T = self.k_T
+        knots, dcoefs, dmax = T.device_rep_arrays(self.dev)
+        out = torch.empty(3 * nat, dtype=torch.float64, device=self.dev)
+        _lib.call("dftb_grad_vrep", _p(self.k_pos), _p(self.k_type), _p(knots), _p(dcoefs), _p(dmax), T.npair,
+                  T.rep_npts, nat, _p(out), _stream())
         return out.cpu().numpy()
 
     def _phi_F(self, v, M, wgt):
@@ -252,7 +264,7 @@ class Gradients:
         self._phi_F(dD, dD, 0.5)
         self._phi_Flr(dD, dD, -0.25)
         gradE0 = self._pairs()
-        gradVrep = np.zeros(3 * nat)
+        gradVrep = self._grad_vrep()
         if I == 0:
             return gradVrep, gradE0, np.zeros(3 * nat)
         # excitation-energy gradient (ExcGradients.py:536-711)

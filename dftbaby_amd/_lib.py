@@ -75,6 +75,8 @@ _SIGS = {
                                         _vp, _vp, _vp, _vp]),
     "dftb_grad_pairs": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp, _i, _i,
                              _vp, _vp]),
+    "dftb_grad_vrep": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp]),
+    "dftb_slako_overlap12": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _i, _i, _vp, _vp]),
     "dftb_slako_matrices": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _i, _i, _d, _vp, _vp, _vp, _vp]),
     "dftb_trans_charges_engine": (_i, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _i, _i, _i, _i,
                                        _vp, _vp, _vp, _vp, _vp, _sz, _vp]),

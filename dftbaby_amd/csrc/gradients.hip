@@ -130,7 +130,60 @@ __global__ void __launch_bounds__(256) grad_pairs_kernel(
   }
 }
 
+// Gradient of the repulsive energy (ExcGradients.py:181-210): grad[3C + c] =
+// sum over D != C of Vrep'(R_CD) (R_C - R_D)_c / R_CD, Vrep' = 0 at and beyond
+// the table end.  One workgroup per atom C, block reduction as above.
+__global__ void __launch_bounds__(256) grad_vrep_kernel(const double* __restrict__ pos,
+                                                        const int* __restrict__ atom_type,
+                                                        const double* __restrict__ knots,
+                                                        const double* __restrict__ dcoefs,
+                                                        const double* __restrict__ dmax, int npair, int npts, int nat,
+                                                        double* __restrict__ grad) {
+  const int C = blockIdx.x;
+  const int tC = atom_type[C];
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int D = threadIdx.x; D < nat; D += blockDim.x) {
+    if (D == C) continue;
+    const double rx = pos[3 * C] - pos[3 * D], ry = pos[3 * C + 1] - pos[3 * D + 1],
+                 rz = pos[3 * C + 2] - pos[3 * D + 2];
+    const double R = sqrt(rx * rx + ry * ry + rz * rz);
+    const int tD = atom_type[D];
+    const int tmin = tC < tD ? tC : tD, tmax = tC < tD ? tD : tC;
+    const int ip = tmax * (tmax + 1) / 2 + tmin;
+    if (ip >= npair || !(R < dmax[ip])) continue;
+    const double dv = spline_deriv(knots + (size_t)ip * npts, npts, dcoefs + (size_t)ip * npts, R);
+    acc[0] += dv * (rx / R);
+    acc[1] += dv * (ry / R);
+    acc[2] += dv * (rz / R);
+  }
+  __shared__ double part[3][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    double v = acc[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) part[q][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int q = threadIdx.x;
+    double s = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += part[q][k];
+    grad[3 * C + q] = s;
+  }
+}
+
 }  // namespace
+
+extern "C" int dftb_grad_vrep(const double* pos, const int* atom_type, const double* knots, const double* dcoefs,
+                              const double* dmax, int npair, int npts, int nat, double* grad, void* stream) {
+  if (nat <= 0 || npair <= 0 || npts < 2 * (slako_dev::kDeg + 1) + 2) return DFTB_ERR_ARG;
+  if (!pos || !atom_type || !knots || !dcoefs || !dmax || !grad) return DFTB_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  grad_vrep_kernel<<<nat, 256, 0, st>>>(pos, atom_type, knots, dcoefs, dmax, npair, npts, nat, grad);
+  DFTB_CHECK_LAUNCH();
+  return DFTB_OK;
+}
 
 extern "C" int dftb_grad_pairs(const double* pos, const int* atom_type, const int* atom_off, const int* orb_l,
                                const int* orb_m, const double* sigma, double rlr, const double* knots,

@@ -108,7 +108,69 @@ __global__ void __launch_bounds__(256) slako_kernel(const double* __restrict__ p
   }
 }
 
+// Overlap between the orbitals of two geometries of one molecule, as
+// slako.f90:576-687 (overlap12) forms it: element (mu, nu), mu on atom i of
+// geometry 1, nu on atom j of geometry 2, direction from R1_i to R2_j, the
+// orbital on the lighter type first; no proximity cutoff and no mirroring.
+__global__ void __launch_bounds__(256) overlap12_kernel(const double* __restrict__ pos1,
+                                                        const double* __restrict__ pos2,
+                                                        const int* __restrict__ orb_atom,
+                                                        const int* __restrict__ orb_l, const int* __restrict__ orb_m,
+                                                        const int* __restrict__ atom_type, int norb,
+                                                        const double* __restrict__ knots,
+                                                        const double* __restrict__ coefs, int npair, int npts,
+                                                        double* __restrict__ S) {
+  const int nu = blockIdx.x * blockDim.x + threadIdx.x;
+  const int mu = blockIdx.y;
+  if (nu >= norb) return;
+  const int i = orb_atom[mu], j = orb_atom[nu];
+  const double xc = pos2[3 * j] - pos1[3 * i], yc = pos2[3 * j + 1] - pos1[3 * i + 1],
+               zc = pos2[3 * j + 2] - pos1[3 * i + 2];
+  const double r = sqrt(xc * xc + yc * yc + zc * zc);
+  double e[3];
+  if (r > 0.0) {
+    e[0] = xc / r; e[1] = yc / r; e[2] = zc / r;
+  } else {
+    e[0] = 0.0; e[1] = 0.0; e[2] = 1.0;
+  }
+  const int ti = atom_type[i], tj = atom_type[j];
+  const bool swap = ti > tj;
+  const int tmin = swap ? tj : ti, tmax = swap ? ti : tj;
+  const int ip = tmax * (tmax + 1) / 2 + tmin;
+  double v = __longlong_as_double(0x7ff8000000000000LL);
+  const int first = swap ? nu : mu, second = swap ? mu : nu;
+  const int l1 = orb_l[first], m1 = orb_m[first], l2 = orb_l[second], m2 = orb_m[second];
+  if (ip < npair && l1 <= 1 && l2 <= 1) {
+    if (swap) { e[0] = -e[0]; e[1] = -e[1]; e[2] = -e[2]; }
+    const double* t = knots + (size_t)ip * npts;
+    const double* c = coefs + (size_t)ip * SLAKO_NTAB * npts;
+    const Basis b = spline_basis(t, npts, r);
+    double T[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) T[k] = 0.0;
+    const int need0 = (l1 == 0 && l2 == 0) ? 0 : (l1 == 0 ? 2 : (l2 == 0 ? 4 : 5));
+    const int need1 = (l1 == 1 && l2 == 1) ? 6 : need0;
+    T[need0] = spline_value(c + (size_t)need0 * npts, b);
+    T[need1] = spline_value(c + (size_t)need1 * npts, b);
+    v = sk_rule(l1, m1, l2, m2, e, T);
+  }
+  S[(size_t)mu * norb + nu] = v;
+}
+
 }  // namespace
+
+extern "C" int dftb_slako_overlap12(const double* pos1, const double* pos2, const int* orb_atom, const int* orb_l,
+                                    const int* orb_m, const int* atom_type, int nat, int norb, const double* knots,
+                                    const double* coefs, int npair, int npts, double* S, void* stream) {
+  if (nat <= 0 || norb <= 0 || npair <= 0 || npts < 2 * (slako_dev::kDeg + 1) + 2) return DFTB_ERR_ARG;
+  if (!pos1 || !pos2 || !orb_atom || !orb_l || !orb_m || !atom_type || !knots || !coefs || !S) return DFTB_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  dim3 grid(ceil_div(norb, 256), norb);
+  overlap12_kernel<<<grid, 256, 0, st>>>(pos1, pos2, orb_atom, orb_l, orb_m, atom_type, norb, knots, coefs, npair,
+                                         npts, S);
+  DFTB_CHECK_LAUNCH();
+  return DFTB_OK;
+}
 
 extern "C" int dftb_slako_matrices(const double* pos, const int* orb_atom, const int* orb_l, const int* orb_m,
                                    const int* atom_type, const double* orben, int nat, int norb,

@@ -21,8 +21,9 @@ reference's "homegrown" parameters (converted by ``make_slako_data``):
 * The Gaussian overlap Omega_AB for the Lambda2 diagnostic
   (``DFTB/DFTB2.py:1287-1317``).
 
-The repulsive potential is not included: excitation energies do not depend on
-it (``DFTB/DFTB2.py:100-102``).
+* The repulsive energy from the tabulated pair potentials
+  (``DFTB/DFTB2.py:2340-2359``, ``RepulsivePotential.py:80-101``); getEnergy
+  returns the total energy E_tot = E_elec + E_rep as the reference's does.
 """
 import math
 import os
@@ -225,7 +226,11 @@ class DFTB2:
             E += self.E_HF_x
         self.en_elec = E
         self.E_elec = E
-        return E
+        # nuclear repulsion (DFTB2.py:2340-2359) and the total energy (DFTB2.py:2274-2275)
+        self.E_rep = slako.repulsive_energy(self.Z, self.pos)
+        self.E_nuc = self.E_rep
+        self.E_tot = E + self.E_rep
+        return self.E_tot
 
     # -- getters used by the excited-state path (DFTB/DFTB2.py:2405-2454) --------
     def getKSCoefficients(self):

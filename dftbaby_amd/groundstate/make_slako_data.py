@@ -2,7 +2,8 @@
 
 This is a build-time data step, not part of the solver.  It reads the
 Slater-Koster tables ``DFTB/SlaterKoster/slako_tables/{h_h,h_c,c_c}.py`` (overlap
-S, hamiltonian H and dipole integrals, keyed (l1, l2, table index)) as
+S, hamiltonian H and dipole integrals, keyed (l1, l2, table index)) and the
+repulsive potentials ``DFTB/RepulsivePotential/reppot_tables/{h_h,h_c,c_c}.py`` as
 *data* (the files are generated modules holding ``array([...])`` literals; they
 are parsed with ``ast`` and never imported or executed) and the free
 pseudo-atom orbital energies of ``DFTB/SlaterKoster/free_pseudo_atoms/{h,c}.py``
@@ -52,6 +53,13 @@ def main(refroot):
             keys = sorted(t[kind].keys())
             data["%s_%s_keys" % (pair, kind)] = np.array(keys, dtype=np.int64)
             data["%s_%s_vals" % (pair, kind)] = np.stack([t[kind][k] for k in keys])
+    # repulsive potentials DFTB/RepulsivePotential/reppot_tables/{h_h,h_c,c_c}.py
+    # (Z1, Z2, d, Vrep), splined by RepulsivePotential.py:80-86
+    repdir = os.path.join(refroot, "DFTB", "RepulsivePotential", "reppot_tables")
+    for pair in ("h_h", "h_c", "c_c"):
+        t = parse_table(os.path.join(repdir, pair + ".py"), {"Z1", "Z2", "d", "Vrep"})
+        data["%s_rep_d" % pair] = np.asarray(t["d"], dtype=np.float64)
+        data["%s_rep_V" % pair] = np.asarray(t["Vrep"], dtype=np.float64)
     for el in ("h", "c"):
         t = parse_table(os.path.join(atdir, el + ".py"),
                         {"energy_1s", "energy_2s", "energy_2p"})

@@ -69,9 +69,39 @@ class Tables:
                 c = coefs[ip, row]
                 diff = c[1:nk1] - c[:nk1 - 1]
                 self.dcoefs[ip, row, :nk1 - 1] = np.where(fac > 0.0, 3.0 * diff / np.where(fac > 0.0, fac, 1.0), 0.0)
+        # repulsive potentials (RepulsivePotential.py:80-101): cubic interpolating
+        # spline of the table, zero at and beyond its last point
+        self.rep_tck, self.rep_dmax = {}, {}
+        rk, rc = [], []
+        for (z1, z2), name in _PAIRS.items():
+            i, j = ATOM_TYPE[z1], ATOM_TYPE[z2]
+            ip = j * (j + 1) // 2 + i
+            tck = splrep(d["%s_rep_d" % name], d["%s_rep_V" % name], s=0)
+            self.rep_tck[ip], self.rep_dmax[ip] = tck, float(d["%s_rep_d" % name][-1])
+        rn = max(len(tck[0]) for tck in self.rep_tck.values())
+        self.rep_npts = rn
+        self.rep_knots = np.zeros((self.npair, rn))
+        self.rep_dcoefs = np.zeros((self.npair, rn))
+        self.rep_dmax_arr = np.zeros(self.npair)
+        for ip, (t, c, k) in self.rep_tck.items():
+            if len(t) != rn:
+                raise ValueError("repulsive tables of one length expected")
+            nk1 = rn - 4
+            fac = t[np.arange(nk1 - 1) + 4] - t[np.arange(nk1 - 1) + 1]
+            self.rep_knots[ip] = t
+            self.rep_dcoefs[ip, :nk1 - 1] = 3.0 * (c[1:nk1] - c[:nk1 - 1]) / fac
+            self.rep_dmax_arr[ip] = self.rep_dmax[ip]
         self.orbital_energies = {1: [float(d["h_energy_1s"])],
                                  6: [float(d["c_energy_2s"])] + 3 * [float(d["c_energy_2p"])]}
         self._dev = {}
+
+    def device_rep_arrays(self, dev):
+        import torch
+        key = "rep" + str(dev)
+        if key not in self._dev:
+            self._dev[key] = tuple(torch.as_tensor(a, device=dev).contiguous()
+                                   for a in (self.rep_knots, self.rep_dcoefs, self.rep_dmax_arr))
+        return self._dev[key]
 
     def device_arrays(self, dev):
         import torch
@@ -281,3 +311,22 @@ def matrices_host(Z, pos, distance_cutoff=30.0, dipoles=True):
             Dm[low[1][blk], low[0][blk]] = Dm[low[0][blk], low[1][blk]]
         D = np.ascontiguousarray(np.moveaxis(D, 0, 2))
     return S, H, D
+
+
+def repulsive_energy(Z, pos):
+    """E_rep = sum over atom pairs of Vrep(R) (DFTB2.py:2340-2359)."""
+    from scipy.interpolate import splev
+    T = tables()
+    Z = np.asarray([int(z) for z in Z])
+    pos = np.asarray(pos, dtype=np.float64).reshape(len(Z), 3)
+    iA, iB = np.tril_indices(len(Z), -1)
+    R = np.linalg.norm(pos[iA] - pos[iB], axis=1)
+    ta = np.array([ATOM_TYPE[z] for z in Z])
+    tmin, tmax = np.minimum(ta[iA], ta[iB]), np.maximum(ta[iA], ta[iB])
+    ip = tmax * (tmax + 1) // 2 + tmin
+    E = 0.0
+    for p, tck in T.rep_tck.items():
+        m = (ip == p) & (R < T.rep_dmax[p])
+        if m.any():
+            E += float(np.sum(splev(R[m], tck)))
+    return E

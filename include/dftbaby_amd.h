@@ -339,6 +339,14 @@ int dftb_mlp_normalize(float* X, long rows, int cols, const float* mean, const f
  * Elements of an unsupported input (l > 1, pair index >= npair) are NaN.
  * H0 = NULL skips H0; the matrices are exactly symmetric. */
 enum { SLAKO_NTAB = 29 };
+/* Overlap S[mu][nu] = <mu(geometry 1)|nu(geometry 2)> between the valence
+ * orbitals of two geometries of one molecule (pos1, pos2 [nat][3]): replaces the
+ * f2py slako.overlap12 (slako.f90:576-687, called from
+ * ScalarCoupling.py:101-131).  Same tables and conventions as
+ * dftb_slako_matrices; no proximity cutoff, not symmetric. */
+int dftb_slako_overlap12(const double* pos1, const double* pos2, const int* orb_atom, const int* orb_l,
+                         const int* orb_m, const int* atom_type, int nat, int norb, const double* knots,
+                         const double* coefs, int npair, int npts, double* S, void* stream);
 int dftb_slako_matrices(const double* pos, const int* orb_atom, const int* orb_l, const int* orb_m,
                         const int* atom_type, const double* orben, int nat, int norb, const double* knots,
                         const double* coefs, int npair, int npts, double cutoff, double* S, double* H0, double* D,
@@ -362,6 +370,14 @@ int dftb_grad_pairs(const double* pos, const int* atom_type, const int* atom_off
                     const double* sigma, double rlr, const double* knots, const double* coefs, const double* dcoefs,
                     int npair, int npts, double cutoff, const double* AS, const double* AH, const double* BG,
                     const double* BL, int nat, int norb, double* grad, void* stream);
+
+/* Gradient of the repulsive energy, replaces ExcGradients.gradient_Vrep
+ * (ExcGradients.py:181-210): grad[3C + c] = sum_{D != C} Vrep'(R_CD)
+ * (R_C - R_D)_c / R_CD with Vrep' = 0 for R >= dmax[pair] (RepulsivePotential.py
+ * getVrep).  knots [npair][npts], dcoefs [npair][npts] the first-derivative
+ * coefficients of the cubic B-spline (as for dftb_grad_pairs), dmax [npair]. */
+int dftb_grad_vrep(const double* pos, const int* atom_type, const double* knots, const double* dcoefs,
+                   const double* dmax, int npair, int npts, int nat, double* grad, void* stream);
 
 #ifdef __cplusplus
 }

@@ -21,10 +21,13 @@ term and in the same tensor layouts (derivative index k = 3 A + c first):
   ``Solver.py:921-1012``), W and the assembly of the excitation-energy gradient
   (``:590-711``), all over the full orbital space (lc = 1, the default "new" lc
   implementation: F^lr of D - D0).
-* ``gradient_Vrep`` is not part of this oracle (the bundled ground state has no
-  repulsive potential).
+* the repulsive energy and its gradient (``DFTB2.py:2340-2359``,
+  ``ExcGradients.py:181-210``) with the splines of ``RepulsivePotential.py:80-101``
+  (``splrep(d, Vrep, s=0)``, zero at and beyond the last table point).
 """
+
 import numpy as np
+from scipy.interpolate import splev, splrep
 from scipy.special import erf
 
 from . import slako_oracle as K
@@ -127,6 +130,49 @@ def h0_and_s_exact(Z, pos, tables=None, distance_cutoff=30.0):
         return K.h0_and_s(Z, pos, tables, distance_cutoff)
     finally:
         K.splev_uniform = saved
+
+
+# ----------------------------------------------------------------------------- Vrep
+def _rep_tables():
+    d = np.load(K._DATA)
+    out = {}
+    for (z1, z2), name in K._PAIRS.items():
+        x = d["%s_rep_d" % name]
+        out[(z1, z2)] = (splrep(x, d["%s_rep_V" % name], s=0), x[-1])
+    return out
+
+
+def _vrep(tabs, z1, z2, R, deriv=0):
+    tck, dmax = tabs[(min(z1, z2), max(z1, z2))]
+    return 0.0 if R >= dmax else float(splev(R, tck, der=deriv))
+
+
+def repulsive_energy(Z, pos):
+    """DFTB2.getRepulsiveEnergy (DFTB2.py:2340-2359)."""
+    tabs = _rep_tables()
+    Z = [int(z) for z in Z]
+    pos = np.asarray(pos, dtype=np.float64)
+    E = 0.0
+    for i in range(1, len(Z)):
+        for j in range(i):
+            E += _vrep(tabs, Z[i], Z[j], np.linalg.norm(pos[i] - pos[j]))
+    return E
+
+
+def gradient_vrep(Z, pos):
+    """gradient_Vrep (ExcGradients.py:181-210)."""
+    tabs = _rep_tables()
+    Z = [int(z) for z in Z]
+    pos = np.asarray(pos, dtype=np.float64)
+    grad = np.zeros(3 * len(Z))
+    for i in range(len(Z)):
+        for j in range(len(Z)):
+            if i == j:
+                continue
+            R = pos[i] - pos[j]
+            Rij = np.linalg.norm(R)
+            grad[3 * i:3 * i + 3] += _vrep(tabs, Z[i], Z[j], Rij, deriv=1) * R / Rij
+    return grad
 
 
 # ----------------------------------------------------------------------------- gamma

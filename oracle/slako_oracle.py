@@ -296,3 +296,29 @@ def dipole_matrix(Z, pos, S, tables=None, distance_cutoff=30.0):
         Dc = D[:, :, c]
         Dc[iu] = Dc.T[iu]
     return D
+
+
+def overlap12(Z, pos1, pos2, tables=None):
+    """<mu(geometry 1)|nu(geometry 2)> as slako.f90:576-687 (overlap12) forms it:
+    direction from atom i of geometry 1 to atom j of geometry 2, lighter type
+    first, no cutoff, no mirroring."""
+    tabs, _ = tables or load_tables()
+    Z = [int(z) for z in Z]
+    p1 = np.asarray(pos1, dtype=np.float64)
+    p2 = np.asarray(pos2, dtype=np.float64)
+    orbs = _orbitals(Z)
+    n = len(orbs)
+    S = np.zeros((n, n))
+    cache = {}
+    for mu in range(n):
+        for nu in range(n):
+            (i, zi, l1, m1), (j, zj, l2, m2) = orbs[mu], orbs[nu]
+            xc = p2[j] - p1[i]
+            r = float(np.sqrt(xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2]))
+            e = xc / r if r > 0.0 else np.array([0.0, 0.0, 1.0])
+            T = _pair_values(tabs, zi, zj, "S", r, cache)
+            if zi <= zj:
+                S[mu, nu] = _sk(l1, m1, l2, m2, e, T)
+            else:
+                S[mu, nu] = _sk(l2, m2, l1, m1, -e, T)
+    return S

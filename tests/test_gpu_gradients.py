@@ -1,8 +1,9 @@
 """GPU analytic LC-TD-DFTB gradients (dftbaby_amd.ExcGradients) against the oracle
 (run with -m gpu).
 
-* C8H10 and C12H14 with displaced atoms: ground-state and excitation-energy
-  gradients of the lowest states against oracle/gradients_oracle.gradient_lc
+* C8H10 and C12H14 with displaced atoms: repulsive, ground-state and
+  excitation-energy gradients of the lowest states against
+  oracle/gradients_oracle (gradient_Vrep, gradient_lc:
   (the reference's tensor formulas over [3 Nat][Norb][Norb] derivative arrays,
   pinned on the CPU by the Fortran dS/dR, dH0/dR and by finite differences).
 * A restricted active space: the amplitudes are padded to the full space as the
@@ -30,7 +31,8 @@ def test_gradients_match_oracle(name, seed):
     grads = Gradients(td)
     for I in (0, 1, 2):
         gV, gE0, gExc = grads.gradient(I=I)
-        assert np.all(gV == 0.0)
+        rV = G.gradient_vrep([z for z, _ in gs.getGeometry()], np.array([x for _, x in gs.getGeometry()]))
+        np.testing.assert_allclose(gV, rV, rtol=0, atol=1e-12 * max(1.0, np.abs(rV).max()))
         if I == 0:
             rE0, _, _ = G.gradient_lc(gs, td.XmY[0], td.XpY[0], td.Omega[0])
             assert np.all(gExc == 0.0)

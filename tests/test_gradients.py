@@ -54,9 +54,9 @@ def _exact_ground_state_class():
 
 def _solve(cls, atoms):
     gs = cls(atoms, long_range_correction=1, long_range_radius=3.03, device="cpu")
-    E = gs.getEnergy(scf_conv=1e-14, maxiter=500)
+    gs.getEnergy(scf_conv=1e-14, maxiter=500)
     ref = O.solve_from_ground_state(gs, nstates=None)
-    return gs, E, ref
+    return gs, gs.E_elec, ref
 
 
 def _fd_sk_tensors(Z, pos, h=1e-5):
@@ -123,3 +123,26 @@ def test_planted_mistake_in_flr_breaks_finite_differences(fd_case, monkeypatch):
     _, gExc, _ = G.gradient_lc(gs, ref["XmY"][0], ref["XpY"][0], ref["Omega"][0], gradS=gS, gradH0=gH)
     fdW = np.array([fd[k][1][0] for k in sorted(fd)])
     assert np.abs(gExc - fdW).max() > 1e-4
+
+
+def test_repulsive_gradient_finite_differences_and_producer():
+    """gradient_Vrep against central differences of the repulsive energy (a
+    displaced C8H10: C-C, C-H and H-H pairs inside and beyond the table ends);
+    the producer's E_rep against the oracle's."""
+    from dftbaby_amd.groundstate import geometry as g
+    from dftbaby_amd.groundstate import slako as SK
+    mol = g.displace(g.polyene(8), 0.08, 11)
+    Z = [z for z, _ in mol]
+    pos = np.array([x for _, x in mol])
+    grad = G.gradient_vrep(Z, pos)
+    h = 1e-5
+    fd = np.zeros_like(grad)
+    for k in range(len(grad)):
+        A, c = divmod(k, 3)
+        p1, p2 = pos.copy(), pos.copy()
+        p1[A, c] += h
+        p2[A, c] -= h
+        fd[k] = (G.repulsive_energy(Z, p1) - G.repulsive_energy(Z, p2)) / (2 * h)
+    np.testing.assert_allclose(grad, fd, rtol=0, atol=1e-8)
+    assert np.abs(grad).max() > 1e-2
+    assert abs(SK.repulsive_energy(Z, pos) - G.repulsive_energy(Z, pos)) < 1e-12

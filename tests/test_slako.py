@@ -132,6 +132,44 @@ def f90_gradients(lib, Z, pos, tabs, cutoff=30.0):
     return np.array(gS), np.array(gH)
 
 
+def f90_overlap12(lib, Z, pos1, pos2, tabs):
+    """<mu(1)|nu(2)> from the reference Fortran (slako.f90 overlap12)."""
+    types, filled, _, arr, (npair, nsk, _, npts) = _f90_tables(tabs)
+    Z = [int(z) for z in Z]
+    nat = len(Z)
+    ind, typ, ls, ms = [], [], [], []
+    for A, z in enumerate(Z):
+        for (l, m) in K.VALORBS[z]:
+            ind.append(A)
+            typ.append(types[z])
+            ls.append(l)
+            ms.append(m)
+    norb = len(ind)
+    I = lambda a: np.asfortranarray(np.asarray(a, dtype=np.int32))
+    ind, typ, ls, ms = I(ind), I(typ), I(ls), I(ms)
+    P1 = np.asfortranarray(np.asarray(pos1, dtype=np.float64).T)
+    P2 = np.asfortranarray(np.asarray(pos2, dtype=np.float64).T)
+    r, x, y, z = (np.zeros((nat, nat), order="F") for _ in range(4))
+    ip = lambda v: ctypes.byref(ctypes.c_int(v))
+    dp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib._QMslakoPdirectional_cosines(dp(P1), dp(P2), ip(nat), ip(nat), dp(r), dp(x), dp(y), dp(z))
+    S = np.zeros((norb, norb), order="F")
+    lib._QMslakoPoverlap12(ip(norb), ip(norb), ip(nat), ip(nat), dp(ind), dp(typ), dp(ls), dp(ms), dp(ind), dp(typ),
+                           dp(ls), dp(ms), dp(r), dp(x), dp(y), dp(z), ip(3), dp(filled), dp(arr["S_knots"]),
+                           dp(arr["S_coefs"]), ip(npair), ip(nsk), ip(npts), dp(S))
+    return np.array(S)
+
+
+def overlap12_geometries():
+    """(name, Z, pos1, pos2): displaced C8H10 and the same moved by a small MD-like step."""
+    from dftbaby_amd.groundstate import geometry as g
+    mol = g.displace(g.polyene(8), 0.05, 21)
+    Z = [a for a, _ in mol]
+    p1 = np.array([x for _, x in mol])
+    p2 = p1 + np.random.default_rng(4).normal(0.0, 0.01, p1.shape)
+    return [("c8h10_step", Z, p1, p2)]
+
+
 def gradient_geometries():
     """(name, Z, pos) of the gradient golden cases: displaced ethylene, and two
     ethylenes 14 bohr apart (table ends, clamped derivatives)."""
