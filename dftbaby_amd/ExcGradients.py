@@ -27,8 +27,9 @@ What differs is how they are formed (identical results to rounding):
   bracket) with L = [C; Y], R = [Y; C] and Gamma_2 the 2 x 2 block repetition
   of gamma_lr on orbital pairs - no q_oo / q_vv.
 
-Only lc = 1 (``gradient_lc``) is built; ``gradient_nolc`` raises
-NotImplementedError.  gradVrep (``gradient_Vrep``, ``:181-210``) comes from the
+Without the long-range correction ``gradient_nolc`` (``:713-981``) is the same
+computation with gamma_lr = 0 (no F^lr terms, H^+ reduced to its Coulomb term, no
+H^- terms, the lc = 0 (A+B) operator).  gradVrep (``gradient_Vrep``, ``:181-210``) comes from the
 ``dftb_grad_vrep`` kernel on the repulsive-potential splines (their derivative
 is clamped below the first table point, where scipy would extrapolate; atoms
 that close do not occur).
@@ -69,8 +70,16 @@ class Gradients:
         self.I = None
 
     def gradient(self, I=0, check_zvector=0, save_intermediates_CPKS=0):
-        if self.dftb.long_range_correction != 1:
-            raise NotImplementedError("gradients without the long-range correction (gradient_nolc) are not built")
+        """(gradVrep, gradE0, gradExc) of state I (0 = ground state), through
+        gradient_lc or, without the long-range correction, gradient_nolc
+        (ExcGradients.py:234-285)."""
+        if self.dftb.long_range_correction == 0:
+            return self.gradient_nolc(I=I, check_zvector=check_zvector, save_intermediates_CPKS=save_intermediates_CPKS)
+        return self.gradient_lc(I=I, check_zvector=check_zvector, save_intermediates_CPKS=save_intermediates_CPKS)
+
+    def gradient_nolc(self, I=0, check_zvector=0, save_intermediates_CPKS=0):
+        """ExcGradients.py:713-981: the terms of gradient_lc without gamma_lr (no
+        F^lr, H^+ reduced to its Coulomb term, no H^- terms, the lc = 0 (A+B))."""
         return self.gradient_lc(I=I, check_zvector=check_zvector, save_intermediates_CPKS=save_intermediates_CPKS)
 
     # ------------------------------------------------------------------ pieces
@@ -99,8 +108,9 @@ class Gradients:
         self.Yo = _mm(self.S, self.Co)
         self.Yv = _mm(self.S, self.Cv)
         am = torch.as_tensor(self.amap, device=dev)
+        self.lc = int(gs.long_range_correction) == 1
         self.g = t(gs.getGamma())
-        self.glr = t(gs.getGamma_lr())
+        self.glr = t(gs.getGamma_lr()) if self.lc else torch.zeros_like(self.g)
         self.gAO = self.g[am][:, am].contiguous()
         self.GAO = self.glr[am][:, am].contiguous()
         self.G2 = self.GAO.repeat(2, 2).contiguous()
@@ -134,7 +144,7 @@ class Gradients:
         _lib.call("dftb_grad_pairs", _p(self.k_pos), _p(self.k_type), _p(self.k_off), _p(self.k_l), _p(self.k_m),
                   _p(self.k_sigma), ctypes.c_double(float(gs.long_range_radius)), _p(self.k_knots),
                   _p(self.k_coefs), _p(self.k_dcoefs), self.k_T.npair, self.k_T.npts, ctypes.c_double(cutoff),
-                  _p(self.AS), _p(self.AH), _p(self.BG), _p(self.BL), nat, n, _p(out), _stream())
+                  _p(self.AS), _p(self.AH), _p(self.BG), _p(self.BL) if self.lc else None, nat, n, _p(out), _stream())
         return out.cpu().numpy()
 
     def _grad_vrep(self):
@@ -198,6 +208,8 @@ class Gradients:
         uu = _mm(self.gAO, u)
         t1 = _mm(Cp, uu * Yq, ta=True, alpha=2.0)
         _mm(Yp, uu * Cq, ta=True, alpha=2.0, out=t1, beta=1.0)
+        if not self.lc:
+            return t1, 0.0, 0.0
         Lp = torch.cat([Cp, Yp], 0)
         Lq = torch.cat([Cq, Yq], 0)
         Rr = torch.cat([Yr, Cr], 0)
@@ -243,8 +255,10 @@ class Gradients:
         gs = self.dftb
         orbs, orbe = gs.getKSCoefficients(), gs.getKSEnergies()
         omega = orbe[self.virt][None, :] - orbe[self.occ][:, None]
+        lc = int(gs.long_range_correction)
+        glr = gs.getGamma_lr() if lc else 0.0 * gs.getGamma()
         return CasidaOperatorGPU.from_orbitals(orbs[:, self.occ], orbs[:, self.virt], gs.getOverlapMatrix(),
-                                               gs.orbsPerAtom, omega, gs.getGamma(), gs.getGamma_lr(), 1)
+                                               gs.orbsPerAtom, omega, gs.getGamma(), glr, lc)
 
     # ------------------------------------------------------------------ gradient
     def gradient_lc(self, I=0, check_zvector=0, save_intermediates_CPKS=0):
@@ -262,7 +276,8 @@ class Gradients:
         self.AH.copy_(D)
         self.AS.sub_(Den)
         self._phi_F(dD, dD, 0.5)
-        self._phi_Flr(dD, dD, -0.25)
+        if self.lc:
+            self._phi_Flr(dD, dD, -0.25)
         gradE0 = self._pairs()
         gradVrep = self._grad_vrep()
         if I == 0:
@@ -290,11 +305,13 @@ class Gradients:
         Gij = self._Hp("o", "o", "v", "v", Tab) - self._Hp("o", "o", "o", "o", Tij)
         Qij = Omega * Uij - Vij + Gij
         Qia = _mm(Xp, self._Hp("v", "v", "o", "v", Xp), tb=True)
-        _mm(Xm, self._Hm("v", "v", "o", "v", Xm), tb=True, out=Qia, beta=1.0)
+        if self.lc:
+            _mm(Xm, self._Hm("v", "v", "o", "v", Xm), tb=True, out=Qia, beta=1.0)
         Qia += self._Hp("o", "v", "v", "v", Tab)
         Qia -= self._Hp("o", "v", "o", "o", Tij)
         Qai = _mm(Xp, self._Hp("o", "o", "o", "v", Xp), ta=True)
-        _mm(Xm, self._Hm("o", "o", "o", "v", Xm), ta=True, out=Qai, beta=1.0)
+        if self.lc:
+            _mm(Xm, self._Hm("o", "o", "o", "v", Xm), ta=True, out=Qai, beta=1.0)
         Qab = Omega * Uab + Vab
         Ria = Qai.T - Qia
         # Z-vector equation (ExcGradients.py:620-640)
@@ -326,9 +343,10 @@ class Gradients:
         self.AH.copy_(M)
         self.AS.sub_(Wao)
         self._phi_F(dD, M, 1.0)
-        self._phi_Flr(dD, M, -0.5)
         self._phi_F(XpYao, XpYao, 2.0)
-        self._phi_Flr(XpYao + XpYao.T, XpYao, -0.5)
-        self._phi_Flr(XmYao - XmYao.T, XmYao, 0.5)
+        if self.lc:
+            self._phi_Flr(dD, M, -0.5)
+            self._phi_Flr(XpYao + XpYao.T, XpYao, -0.5)
+            self._phi_Flr(XmYao - XmYao.T, XmYao, 0.5)
         gradExc = self._pairs()
         return gradVrep, gradE0, gradExc

@@ -20,7 +20,8 @@ term and in the same tensor layouts (derivative index k = 3 A + c first):
   H^+ / H^- contractions, Q, the Z-vector equation (Krylov solver of
   ``Solver.py:921-1012``), W and the assembly of the excitation-energy gradient
   (``:590-711``), all over the full orbital space (lc = 1, the default "new" lc
-  implementation: F^lr of D - D0).
+  implementation: F^lr of D - D0); ``gradient_nolc`` (``:713-981``) is the same
+  with gamma_lr = 0, which removes F^lr, the exchange terms of H^+ and H^-.
 * the repulsive energy and its gradient (``DFTB2.py:2340-2359``,
   ``ExcGradients.py:181-210``) with the splines of ``RepulsivePotential.py:80-101``
   (``splrep(d, Vrep, s=0)``, zero at and beyond the last table point).
@@ -301,13 +302,18 @@ def Hminus(g0lr, q_ps, q_qr, q_pr, q_qs, v_rs):
 def gradient_lc(gs, XmY, XpY, Omega, gradS=None, gradH0=None, exact=False, zconv=1e-14):
     """(gradE0, gradExc, Zia) of ExcGradients.py:286-711 for one excited state with
     amplitudes XmY, XpY (nocc x nvirt over the full orbital space) and energy
-    Omega, on a ground-state object with the DFTB2 getters (lc = 1)."""
+    Omega, on a ground-state object with the DFTB2 getters; for lc = 0 the same
+    formulas with gamma_lr = 0 are gradient_nolc (ExcGradients.py:713-981)."""
     Z = np.array([z for z, _ in gs.getGeometry()])
     pos = np.array([x for _, x in gs.getGeometry()])
     U = gs.hubbard_U
     opa = gs.orbsPerAtom
     g0, _, g0_AO, g1_AO = gamma_aowise(Z, pos, U, opa)
     g0lr, _, g0lr_AO, g1lr_AO = gamma_aowise(Z, pos, U, opa, Rlr=gs.long_range_radius)
+    if gs.long_range_correction == 0:
+        # gradient_nolc (ExcGradients.py:713-981) is gradient_lc with gamma_lr = 0:
+        # no F^lr, H^+ reduced to its Coulomb term, no H^- terms, lc = 0 (A+B)
+        g0lr, g0lr_AO, g1lr_AO = 0.0 * g0lr, 0.0 * g0lr_AO, 0.0 * g1lr_AO
     f = gs.getOccupation()
     orbe = gs.getKSEnergies()
     orbs = gs.getKSCoefficients()

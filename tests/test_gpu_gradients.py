@@ -1,8 +1,9 @@
 """GPU analytic LC-TD-DFTB gradients (dftbaby_amd.ExcGradients) against the oracle
 (run with -m gpu).
 
-* C8H10 and C12H14 with displaced atoms: repulsive, ground-state and
-  excitation-energy gradients of the lowest states against
+* C8H10 and C12H14 with displaced atoms (lc = 1, and C8H10 with lc = 0,
+  gradient_nolc): repulsive, ground-state and excitation-energy gradients of
+  the lowest states against
   oracle/gradients_oracle (gradient_Vrep, gradient_lc:
   (the reference's tensor formulas over [3 Nat][Norb][Norb] derivative arrays,
   pinned on the CPU by the Fortran dS/dR, dH0/dR and by finite differences).
@@ -21,11 +22,11 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-@pytest.mark.parametrize("name,seed", [("C8H10", 23), ("C12H14", 5)])
-def test_gradients_match_oracle(name, seed):
+@pytest.mark.parametrize("name,seed,lc", [("C8H10", 23, 1), ("C12H14", 5, 1), ("C8H10", 23, 0)])
+def test_gradients_match_oracle(name, seed, lc):
     from dftbaby_amd.ExcGradients import Gradients
     from dftbaby_amd.LR_TDDFTB import LR_TDDFTB
-    gs = ground_state(name, 1, 0.05, seed)
+    gs = ground_state(name, lc, 0.05, seed)
     td = LR_TDDFTB(ground_state=gs)
     td.getEnergies(nstates=4, diag_conv=1e-10, diag_maxiter=100)
     grads = Gradients(td)

@@ -3,8 +3,9 @@
 * dS/dR, dH0/dR against DFTBaby's own Fortran (slako.f90 gradients_H0andS built
   into oracle/_ref) when that build exists, and against its golden vectors
   (tests/golden/slako_f90_golden.npz).
-* The whole LC-TD-DFTB gradient (ground state and excitation energies,
-  ExcGradients.py:286-711) against central finite differences of the SCC energy
+* The whole TD-DFTB gradient (ground state and excitation energies,
+  gradient_lc, ExcGradients.py:286-711, and gradient_nolc, :713-981) against
+  central finite differences of the SCC energy
   and of the excitation energies from the dense Casida solve.  Finite differences
   need a smooth energy, so this check runs on a ground state whose S and H0 come
   from the splines themselves (not splev_uniform's shifted interval), with the
@@ -52,8 +53,8 @@ def _exact_ground_state_class():
     return ExactSplineDFTB2
 
 
-def _solve(cls, atoms):
-    gs = cls(atoms, long_range_correction=1, long_range_radius=3.03, device="cpu")
+def _solve(cls, atoms, lc=1):
+    gs = cls(atoms, long_range_correction=lc, long_range_radius=3.03, device="cpu")
     gs.getEnergy(scf_conv=1e-14, maxiter=500)
     ref = O.solve_from_ground_state(gs, nstates=None)
     return gs, gs.E_elec, ref
@@ -77,14 +78,15 @@ def _fd_sk_tensors(Z, pos, h=1e-5):
     return gS, gH
 
 
-@pytest.fixture(scope="module")
-def fd_case():
+@pytest.fixture(scope="module", params=[1, 0], ids=["lc1", "lc0"])
+def fd_case(request):
     from dftbaby_amd.groundstate import geometry as g
+    lc = request.param
     cls = _exact_ground_state_class()
     mol = g.displace(g.ethylene(), 0.05, 7)
     Z = [z for z, _ in mol]
     pos = np.array([x for _, x in mol])
-    gs, E, ref = _solve(cls, mol)
+    gs, E, ref = _solve(cls, mol, lc)
     gS, gH = _fd_sk_tensors(Z, pos)
     h = 1e-4
     fd = {}
@@ -94,7 +96,7 @@ def fd_case():
         for s in (1.0, -1.0):
             p = pos.copy()
             p[A, c] += s * h
-            _, E2, r2 = _solve(cls, [(z, list(x)) for z, x in zip(Z, p)])
+            _, E2, r2 = _solve(cls, [(z, list(x)) for z, x in zip(Z, p)], lc)
             vals.append((E2, r2["Omega"][:3]))
         fd[k] = ((vals[0][0] - vals[1][0]) / (2 * h), (vals[0][1] - vals[1][1]) / (2 * h))
     return gs, ref, gS, gH, fd
@@ -112,6 +114,8 @@ def test_gradient_lc_against_finite_differences(fd_case, state):
 
 def test_planted_mistake_in_flr_breaks_finite_differences(fd_case, monkeypatch):
     gs, ref, gS, gH, fd = fd_case
+    if gs.long_range_correction == 0:
+        pytest.skip("F^lr does not enter without the long-range correction")
     full = G.Flr_op
 
     def without_term12(v, S, gradS, g0lr_AO, g1lr_AO):
