@@ -84,10 +84,17 @@ class DFTB2:
         self.setGeometry(atomlist)
 
     # -- geometry dependent set-up ------------------------------------------------
-    def setGeometry(self, atomlist, **opts):
+    def setGeometry(self, atomlist, keep_charge=False, **opts):
+        """New nuclear positions.  keep_charge=True (as PES.py:62,86 passes it)
+        keeps the converged density of the previous geometry of the same molecule
+        as the SCC's starting point (the reference keeps its partial charges,
+        DFTB2.setGeometry); otherwise the SCC starts from the neutral reference
+        density."""
         for Z, _ in atomlist:
             if Z not in (1, 6):
                 raise ValueError("only H and C are parametrised in this producer")
+        same = getattr(self, "Z", None) is not None and [int(Z) for Z, _ in atomlist] == list(self.Z)
+        self._P_start = self.P if (keep_charge and same and getattr(self, "P", None) is not None) else None
         self.atomlist = [(int(Z), list(map(float, x))) for Z, x in atomlist]
         self.Z = np.array([Z for Z, _ in self.atomlist])
         self.pos = np.array([x for _, x in self.atomlist], dtype=np.float64)
@@ -154,7 +161,7 @@ class DFTB2:
         nocc = self.Nelec_val // 2
         if self.Nelec_val % 2:
             raise ValueError("closed-shell ground state needs an even electron count")
-        P = P0.clone()
+        P = t(self._P_start) if getattr(self, "_P_start", None) is not None else P0.clone()
         fock_hist, err_hist = [], []
         converged = False
         for it in range(maxiter):

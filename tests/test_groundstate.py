@@ -138,3 +138,26 @@ def test_gaussian_overlap_quadrature():
         if D[A, B] < 4.0:
             assert abs(_overlap_gaussians_quadrature(D[A, B], sw2) - ref) > 1e-4 * ref
     np.testing.assert_allclose(Om, Om.T, rtol=0, atol=0)
+
+
+def test_scc_warm_start_from_previous_geometry():
+    """setGeometry(keep_charge=True) starts the SCC from the previous converged
+    density (the reference keeps its charges, DFTB2.setGeometry): the same
+    converged state as a cold start, in fewer iterations."""
+    from dftbaby_amd.groundstate import geometry as g
+    from dftbaby_amd.groundstate.dftb2 import DFTB2
+    mol = g.displace(g.polyene(20), 0.05, 3)
+    gs = DFTB2(mol, long_range_correction=1, device="cpu")
+    gs.getEnergy(scf_conv=1e-10)
+    x = np.array([p for _, p in mol]) + np.random.default_rng(0).normal(0, 0.01, (len(mol), 3))
+    at = [(z, list(p)) for (z, _), p in zip(mol, x)]
+    gs.setGeometry(at, keep_charge=True)
+    E_warm = gs.getEnergy(scf_conv=1e-10)
+    cold = DFTB2(at, long_range_correction=1, device="cpu")
+    E_cold = cold.getEnergy(scf_conv=1e-10)
+    assert abs(E_warm - E_cold) < 1e-9
+    assert np.abs(gs.getKSEnergies() - cold.getKSEnergies()).max() < 1e-7
+    assert gs.scc_iterations < cold.scc_iterations
+    gs.setGeometry(at)                        # without keep_charge: a cold start
+    gs.getEnergy(scf_conv=1e-10)
+    assert gs.scc_iterations == cold.scc_iterations
