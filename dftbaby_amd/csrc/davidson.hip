@@ -172,11 +172,11 @@ extern "C" int dftb_unit_vectors(const long* idx, int k, long n, double* X, void
 // ---------------------------------------------------------------------------
 // Cholesky-QR factor of the Gram matrix G = Q Q^T (m x m) with rank dropping:
 // a row whose pivot falls below tol2 * ref[j] is flagged and zeroed.  Writes
-// Linv = L^{-1} (rows of dropped vectors zero).  One workgroup.
-__global__ __launch_bounds__(256) void cholqr_kernel(const double* __restrict__ G, int m, const double* __restrict__ ref,
-                                                     double tol2, double* __restrict__ L, double* __restrict__ Linv,
-                                                     int* __restrict__ keep) {
-  __shared__ int s_keep;
+// Linv = L^{-1} (rows of dropped vectors zero).  One workgroup; G, L and Linv in
+// LDS when 3 m^2 doubles fit (CHOLQR_LDS_M), else in global memory.
+#define CHOLQR_LDS_M 52
+__device__ void cholqr_core(const double* G, int m, const double* __restrict__ ref, double tol2, double* L,
+                            double* Linv, int* __restrict__ keep, int* s_keep) {
   for (int i = threadIdx.x; i < m * m; i += blockDim.x) {
     L[i] = 0.0;
     Linv[i] = 0.0;
@@ -190,15 +190,16 @@ __global__ __launch_bounds__(256) void cholqr_kernel(const double* __restrict__ 
       // the pivot of a dependent row is Cholesky cancellation noise ~ eps * G_jj:
       // require it to clear both the caller's tolerance and 1e-12 * G_jj
       int ok = keep[j] && (d > tol2 * r) && (d > 1.0e-12 * G[j * m + j]) && (d > 0.0);
-      s_keep = ok;
+      *s_keep = ok;
       keep[j] = ok;
       L[j * m + j] = ok ? sqrt(d) : 1.0;
     }
     __syncthreads();
     const double ljj = L[j * m + j];
+    const int kj = *s_keep;
     for (int i = j + 1 + threadIdx.x; i < m; i += blockDim.x) {
       double s = 0.0;
-      if (s_keep) {
+      if (kj) {
         s = G[i * m + j];
         for (int k = 0; k < j; ++k) s -= L[i * m + k] * L[j * m + k];
         s /= ljj;
@@ -222,6 +223,92 @@ __global__ __launch_bounds__(256) void cholqr_kernel(const double* __restrict__ 
   }
 }
 
+__global__ __launch_bounds__(256) void cholqr_kernel(const double* __restrict__ G, int m, const double* __restrict__ ref,
+                                                     double tol2, double* __restrict__ L, double* __restrict__ Linv,
+                                                     int* __restrict__ keep) {
+  __shared__ int s_keep;
+  cholqr_core(G, m, ref, tol2, L, Linv, keep, &s_keep);
+}
+
+__global__ __launch_bounds__(256) void cholqr_lds_kernel(const double* __restrict__ G, int m,
+                                                         const double* __restrict__ ref, double tol2,
+                                                         double* __restrict__ L, double* __restrict__ Linv,
+                                                         int* __restrict__ keep) {
+  __shared__ double sm[3 * CHOLQR_LDS_M * CHOLQR_LDS_M];
+  __shared__ int s_keep;
+  double* sG = sm;
+  double* sL = sm + m * m;
+  double* sLi = sL + m * m;
+  for (int i = threadIdx.x; i < m * m; i += blockDim.x) sG[i] = G[i];
+  __syncthreads();
+  cholqr_core(sG, m, ref, tol2, sL, sLi, keep, &s_keep);
+  __syncthreads();
+  for (int i = threadIdx.x; i < m * m; i += blockDim.x) {
+    L[i] = sL[i];
+    Linv[i] = sLi[i];
+  }
+}
+
+static void cholqr_launch(const double* G, int m, const double* ref, double tol2, double* L, double* Linv, int* keep,
+                          hipStream_t st) {
+  if (m <= CHOLQR_LDS_M)
+    hipLaunchKernelGGL(cholqr_lds_kernel, dim3(1), dim3(256), 0, st, G, m, ref, tol2, L, Linv, keep);
+  else
+    hipLaunchKernelGGL(cholqr_kernel, dim3(1), dim3(256), 0, st, G, m, ref, tol2, L, Linv, keep);
+}
+
+// dst[i][e] = sum_{k <= i} Linv[i][k] src[k][e] for a lower-triangular m x m Linv
+// (m <= MP): one column e per thread with src[0..m)[e] in registers and Linv in
+// LDS (zero-padded to MP x MP, uniform addresses: broadcast reads).  Streams src
+// once and writes dst once.
+template <int MP>
+__global__ __launch_bounds__(256) void tri_apply_kernel(const double* __restrict__ Linv, int m,
+                                                        const double* __restrict__ src, double* __restrict__ dst,
+                                                        long n) {
+  __shared__ double sL[MP * MP];
+  for (int i = threadIdx.x; i < MP * MP; i += blockDim.x) {
+    const int r = i / MP, c = i % MP;
+    sL[i] = (r < m && c < m) ? Linv[r * m + c] : 0.0;
+  }
+  __syncthreads();
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    double x[MP];
+#pragma unroll
+    for (int k = 0; k < MP; ++k) x[k] = (k < m) ? src[(long)k * n + e] : 0.0;
+    for (int i = 0; i < m; ++i) {
+      const double* li = sL + i * MP;
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < MP; ++k)
+        if (k <= i) s += li[k] * x[k];       // uniform branch: the triangle only
+      dst[(long)i * n + e] = s;
+    }
+  }
+}
+
+// dst = Linv src (Linv m x m lower triangular, src / dst m x n, distinct)
+static int tri_apply(const double* Linv, int m, const double* src, double* dst, long n, hipStream_t st) {
+  long g = (n + 255) / 256;
+  if (g > 8192) g = 8192;
+  const dim3 grid((unsigned)g), blk(256);
+  if (m <= 8) hipLaunchKernelGGL(tri_apply_kernel<8>, grid, blk, 0, st, Linv, m, src, dst, n);
+  else if (m <= 16) hipLaunchKernelGGL(tri_apply_kernel<16>, grid, blk, 0, st, Linv, m, src, dst, n);
+  else if (m <= 24) hipLaunchKernelGGL(tri_apply_kernel<24>, grid, blk, 0, st, Linv, m, src, dst, n);
+  else if (m <= 32) hipLaunchKernelGGL(tri_apply_kernel<32>, grid, blk, 0, st, Linv, m, src, dst, n);
+  else if (m <= 40) hipLaunchKernelGGL(tri_apply_kernel<40>, grid, blk, 0, st, Linv, m, src, dst, n);
+  else if (m <= 48) hipLaunchKernelGGL(tri_apply_kernel<48>, grid, blk, 0, st, Linv, m, src, dst, n);
+  else {
+    GemmArgs u = gemm_args_default();
+    u.A = Linv; u.lda = m;
+    u.B = src; u.ldb = n;
+    u.C = dst; u.ldc = n;
+    u.M = m; u.N = (int)n; u.K = m; u.tile = 64;
+    return gemm_launch(u, st);
+  }
+  DFTB_CHECK_LAUNCH();
+  return DFTB_OK;
+}
+
 __global__ void init_keep_kernel(int* keep, int m) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m) keep[i] = 1;
@@ -232,7 +319,7 @@ static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct OrthoLayout {
   int ks;
   long kchunk;
-  size_t part, C, G, L, Linv, ref, flag, tmp, total;
+  size_t part, C, G, L, Linv, ref, flag, nrm, tmp, total;
 };
 
 static OrthoLayout ortho_layout(int l, int m, long n) {
@@ -257,6 +344,7 @@ static OrthoLayout ortho_layout(int l, int m, long n) {
   o.Linv = off; off += al256((size_t)m * m * 8);
   o.ref = off; off += al256((size_t)m * 8);
   o.flag = off; off += al256(sizeof(int));
+  o.nrm = off; off += al256((size_t)m * NORM_SLICES * 8);
   o.tmp = off; off += al256((size_t)m * n * 8);
   o.total = off;
   return o;
@@ -337,11 +425,6 @@ extern "C" int dftb_gram(const double* X, long ldx, int P, const double* Y, long
   return DFTB_OK;
 }
 
-// ref[j] = sqrt(G_jj)
-__global__ void diag_sqrt_kernel(const double* __restrict__ G, int m, double* __restrict__ ref) {
-  for (int j = threadIdx.x; j < m; j += blockDim.x) ref[j] = sqrt(fmax(G[(long)j * m + j], 0.0));
-}
-
 // flag = 1 when some vector kept less than half of its squared norm through the
 // projection (G_jj < ref_j^2 / 2): the Daniel-Gragg-Kaufman-Stewart test for a
 // second classical Gram-Schmidt pass
@@ -373,9 +456,11 @@ extern "C" int dftb_orthonormalize(const double* basis, int l, double* Q, int m,
   int rc;
   hipLaunchKernelGGL(init_keep_kernel, dim3((m + 255) / 256), dim3(256), 0, st, keep, m);
   DFTB_CHECK_LAUNCH();
-  // reference norms before the projections: the diagonal of Q Q^T
-  if ((rc = gram(Q, m, Q, m, n, o, ws, G, st))) return rc;
-  hipLaunchKernelGGL(diag_sqrt_kernel, dim3(1), dim3(256), 0, st, (const double*)G, m, ref);
+  // reference norms before the projections
+  hipLaunchKernelGGL(row_norms_kernel, dim3(NORM_SLICES, m), dim3(256), 0, st, (const double*)Q, n,
+                     (double*)(ws + o.nrm));
+  hipLaunchKernelGGL(slice_norms_kernel, dim3((m + 63) / 64), dim3(64), 0, st, (const double*)(ws + o.nrm), m,
+                     NORM_SLICES, ref);
   DFTB_CHECK_LAUNCH();
   // classical Gram-Schmidt against the basis; a second pass when the DGKS test
   // asks for it (G = Q Q^T of the projected block serves the test and the first
@@ -398,20 +483,16 @@ extern "C" int dftb_orthonormalize(const double* basis, int l, double* Q, int m,
       if (!hflag) break;
     }
   }
+  // without a basis the first Cholesky-QR needs the block's own Gram matrix
+  if (l == 0 && (rc = gram(Q, m, Q, m, n, o, ws, G, st))) return rc;
   // two passes of Cholesky-QR inside the block: Q -> tmp -> Q
   for (int pass = 0; pass < 2; ++pass) {
     const double* src = pass == 0 ? Q : tmp;
     double* dst = pass == 0 ? tmp : Q;
     if (pass == 1 && (rc = gram(src, m, src, m, n, o, ws, G, st))) return rc;
-    hipLaunchKernelGGL(cholqr_kernel, dim3(1), dim3(256), 0, st, G, m, pass == 0 ? ref : (const double*)nullptr,
-                       pass == 0 ? tol * tol : 1.0e-24, L, Linv, keep);
+    cholqr_launch(G, m, pass == 0 ? ref : (const double*)nullptr, pass == 0 ? tol * tol : 1.0e-24, L, Linv, keep, st);
     DFTB_CHECK_LAUNCH();
-    GemmArgs u = gemm_args_default();
-    u.A = Linv; u.lda = m;
-    u.B = src; u.ldb = n;
-    u.C = dst; u.ldc = n;
-    u.M = m; u.N = (int)n; u.K = m; u.tile = 64;
-    if ((rc = gemm_launch(u, st))) return rc;
+    if ((rc = tri_apply(Linv, m, src, dst, n, st))) return rc;
   }
   return DFTB_OK;
 }

@@ -34,11 +34,20 @@ __global__ __launch_bounds__(EIG_THREADS) void syevx_kernel(const double* __rest
 }
 
 // ---- multi-workgroup Householder reduction (large l) -----------------------
-// Column j is reduced by two launches over all workgroups: hh_mv forms
-// p = t A22 v (one wave per row), hh_up applies A22 -= v w^T + w v^T with
-// w = p - (t/2)(p.v) v (rows spread over waves) and stores v in column j.  Every
-// workgroup rebuilds v from the still unmodified column j in LDS, and the kernel
-// boundary orders the updates, so no grid-wide synchronisation is needed.
+// One launch per column j (hh_step) finishes step j and starts step j + 1:
+//   - every workgroup rebuilds v_j from the updated column j (col, written by the
+//     previous launch) and w = p - (t/2)(p.v) v from p = t A22 v (previous launch);
+//   - every workgroup forms the updated column j + 1 itself (from row j + 1, by
+//     symmetry) and v_{j+1} from it; workgroup 0 stores it for the next launch;
+//   - each wave updates one row of the trailing block, A -= v w^T + w v^T, for the
+//     columns beyond j + 1, and with the values it has just written forms that
+//     row's entry of p' = t' A22' v_{j+1} for the next launch.
+// Nothing reads what the launch writes: column / row j + 1 are never written
+// (their updated values travel in col), p and col alternate between two buffers,
+// and column j below the subdiagonal receives v_j (rows >= 1) for the
+// back-transformation.  The update term (v_r w_c) + (w_r v_c) is evaluated
+// without contraction, so entries (r, c) and (c, r) stay bitwise equal and row
+// j + 1 is column j + 1 exactly.
 #define HH_THREADS 512
 
 __global__ __launch_bounds__(256) void sym_copy_kernel(const double* __restrict__ A, int l, int ld, double* __restrict__ W) {
@@ -48,15 +57,15 @@ __global__ __launch_bounds__(256) void sym_copy_kernel(const double* __restrict_
   }
 }
 
-// Householder vector of column j in LDS: returns t (0 when the column is already
-// zero below the subdiagonal), beta via *beta_out; sv[0..n2) = v.
-__device__ double hh_vector(const double* __restrict__ W, int l, int j, double* sv, double* sh, double* beta_out) {
-  const int n2 = l - j - 1, tid = threadIdx.x;
+// Householder vector of the n2 entries x (global or LDS) into sv: returns t (0 when
+// the entries are all zero), beta via *beta_out.  Ends with the block synchronised.
+__device__ double hh_vec(const double* x, int n2, double* sv, double* sh, double* beta_out) {
+  const int tid = threadIdx.x;
   double part = 0.0;
   for (int r = tid; r < n2; r += blockDim.x) {
-    const double x = W[(long)(j + 1 + r) * l + j];
-    sv[r] = x;
-    part += x * x;
+    const double a = x[r];
+    sv[r] = a;
+    part += a * a;
   }
   const double alpha2 = blk_sum(part, sh);
   const double x0 = sv[0];
@@ -69,24 +78,30 @@ __device__ double hh_vector(const double* __restrict__ W, int l, int j, double* 
   const double beta = (x0 >= 0.0) ? -alpha : alpha;
   const double v0 = x0 - beta;
   const double vtv = alpha2 - x0 * x0 + v0 * v0;
-  __syncthreads();
   if (tid == 0) sv[0] = v0;
   __syncthreads();
   *beta_out = beta;
   return 2.0 / vtv;
 }
 
-__global__ __launch_bounds__(HH_THREADS) void hh_mv_kernel(const double* __restrict__ W, int l, int j,
-                                                           double* __restrict__ P) {
+// (a b) + (c d) rounded without contraction: symmetric in swapping the pairs
+__device__ __forceinline__ double sym_upd(double a, double b, double c, double d) {
+  return __dadd_rn(__dmul_rn(a, b), __dmul_rn(c, d));
+}
+
+// Step 0's inputs: col = row 0 of W (= column 0), P[r] = t0 (A22 v0)_r.
+__global__ __launch_bounds__(HH_THREADS) void hh_start_kernel(const double* __restrict__ W, int l,
+                                                              double* __restrict__ col, double* __restrict__ P) {
   __shared__ double sv[EIG_MAXL];
   __shared__ double sh[HH_THREADS / 64];
-  const int n2 = l - j - 1;
+  const int n2 = l - 1, tid = threadIdx.x;
+  if (blockIdx.x == 0)
+    for (int r = tid; r < l; r += blockDim.x) col[r] = W[r];
   double beta;
-  const double t = hh_vector(W, l, j, sv, sh, &beta);
-  if (t == 0.0) return;
-  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
-  for (int r = blockIdx.x * nw + (threadIdx.x >> 6); r < n2; r += gridDim.x * nw) {
-    const double* row = W + (long)(j + 1 + r) * l + (j + 1);
+  const double t = hh_vec(W + 1, n2, sv, sh, &beta);
+  const int lane = tid & 63, nw = blockDim.x >> 6;
+  for (int r = blockIdx.x * nw + (tid >> 6); r < n2; r += gridDim.x * nw) {
+    const double* row = W + (long)(1 + r) * l + 1;
     double s = 0.0;
     for (int c = lane; c < n2; c += 64) s += row[c] * sv[c];
     s = wave_sum(s);
@@ -94,44 +109,91 @@ __global__ __launch_bounds__(HH_THREADS) void hh_mv_kernel(const double* __restr
   }
 }
 
-// Column j keeps its original entries while the launch runs (other workgroups
-// rebuild v from it): rows r >= 1 of v equal them, and v_0 goes to hv0[j]; the
-// reflectors' first entries are written back after the last column.
-__global__ __launch_bounds__(HH_THREADS) void hh_up_kernel(double* __restrict__ W, int l, int j,
-                                                           const double* __restrict__ P, double* __restrict__ d,
-                                                           double* __restrict__ e, double* __restrict__ tau,
-                                                           double* __restrict__ hv0) {
-  __shared__ double sv[EIG_MAXL];
-  __shared__ double sw[EIG_MAXL];
+// col[0] = updated (j, j), col[1 + r] = updated (j + 1 + r, j); P[r] = t_j (A22 v_j)_r
+// (r < n2).  Writes coln (updated column j + 1 from its diagonal down) and Pn (n2 - 1).
+__global__ __launch_bounds__(HH_THREADS) void hh_step_kernel(double* __restrict__ W, int l, int j,
+                                                             const double* __restrict__ col, double* __restrict__ coln,
+                                                             const double* __restrict__ P, double* __restrict__ Pn,
+                                                             double* __restrict__ d, double* __restrict__ e,
+                                                             double* __restrict__ tau, double* __restrict__ hv0) {
+  __shared__ double sv[EIG_MAXL];   // v_j
+  __shared__ double sw[EIG_MAXL];   // w_j
+  __shared__ double sn[EIG_MAXL];   // v_{j+1}
   __shared__ double sh[HH_THREADS / 64];
   const int n2 = l - j - 1, tid = threadIdx.x;
   double beta;
-  const double t = hh_vector(W, l, j, sv, sh, &beta);
+  const double t = hh_vec(col + 1, n2, sv, sh, &beta);
   if (blockIdx.x == 0 && tid == 0) {
     tau[j] = t;
     e[j] = beta;
-    d[j] = W[(long)j * l + j];
+    d[j] = col[0];
     hv0[j] = (t == 0.0) ? 0.0 : sv[0];
   }
-  const int lane = tid & 63, nw = blockDim.x >> 6;
+  __syncthreads();
   if (t == 0.0) {
-    for (int r = 1 + blockIdx.x * blockDim.x + tid; r < n2; r += gridDim.x * blockDim.x)
-      W[(long)(j + 1 + r) * l + j] = 0.0;
-    return;
+    // nothing to annihilate: no update, and column j below the subdiagonal is zero
+    for (int r = tid; r < n2; r += blockDim.x) {
+      sv[r] = 0.0;
+      sw[r] = 0.0;
+    }
+  } else {
+    double part = 0.0;
+    for (int r = tid; r < n2; r += blockDim.x) {
+      const double pr = P[r];
+      sw[r] = pr;
+      part += pr * sv[r];
+    }
+    const double K = 0.5 * t * blk_sum(part, sh);
+    for (int r = tid; r < n2; r += blockDim.x) sw[r] -= K * sv[r];
   }
+  __syncthreads();
+  // updated column j + 1 = updated row j + 1 (rows j + 1 ..)
+  const double* rowp = W + (long)(j + 1) * l + (j + 1);
+  const double v0 = sv[0], w0 = sw[0];
   double part = 0.0;
   for (int r = tid; r < n2; r += blockDim.x) {
-    const double pr = P[r];
-    sw[r] = pr;
-    part += pr * sv[r];
+    const double a = rowp[r] - sym_upd(v0, sw[r], w0, sv[r]);
+    if (blockIdx.x == 0) coln[r] = a;
+    if (r >= 1) {
+      sn[r - 1] = a;
+      part += a * a;
+    }
   }
-  const double K = 0.5 * t * blk_sum(part, sh);
-  for (int r = tid; r < n2; r += blockDim.x) sw[r] -= K * sv[r];
+  const double alpha2 = blk_sum(part, sh);
+  const double x0 = sn[0];
   __syncthreads();
-  for (int r = blockIdx.x * nw + (tid >> 6); r < n2; r += gridDim.x * nw) {
+  double tn = 0.0;
+  if (alpha2 > 1e-300) {
+    const double alpha = sqrt(alpha2);
+    const double bn = (x0 >= 0.0) ? -alpha : alpha;
+    const double vn0 = x0 - bn;
+    tn = 2.0 / (alpha2 - x0 * x0 + vn0 * vn0);
+    if (tid == 0) sn[0] = vn0;
+  }
+  __syncthreads();
+  const int lane = tid & 63, nw = blockDim.x >> 6;
+  for (int r = 1 + blockIdx.x * nw + (tid >> 6); r < n2; r += gridDim.x * nw) {
     double* row = W + (long)(j + 1 + r) * l + (j + 1);
     const double vr = sv[r], wr = sw[r];
-    for (int c = lane; c < n2; c += 64) row[c] -= vr * sw[c] + wr * sv[c];
+    double s = 0.0;
+    for (int c = 1 + lane; c < n2; c += 64) {
+      const double a = row[c] - sym_upd(vr, sw[c], wr, sv[c]);
+      row[c] = a;
+      s += a * sn[c - 1];
+    }
+    s = wave_sum(s);
+    if (lane == 0) {
+      Pn[r - 1] = tn * s;
+      W[(long)(j + 1 + r) * l + j] = vr;   // v_j below its first entry (0 when t = 0)
+    }
+  }
+}
+
+// The last 2 x 2 block: the updated (l-2, l-2) and (l-1, l-2) from col.
+__global__ void hh_tail_kernel(double* __restrict__ W, int l, const double* __restrict__ col) {
+  if (threadIdx.x == 0) {
+    W[(long)(l - 2) * l + (l - 2)] = col[0];
+    W[(long)(l - 1) * l + (l - 2)] = col[1];
   }
 }
 
@@ -240,18 +302,27 @@ extern "C" int dftb_syevx(const double* A, int l, int ld, int nev, double* evals
   double* d = (double*)(ws + Lo.d);
   double* e = (double*)(ws + Lo.e);
   double* tau = (double*)(ws + Lo.tau);
-  double* P = (double*)(ws + Lo.scratch);   // product vector (the tail's scratch is not yet in use)
-  double* hv0 = P + l;                       // first entries of the reflectors
+  // product vectors and updated columns (two of each, alternating) and the
+  // reflectors' first entries: 5 l doubles of the tail's scratch (not yet in use)
+  double* P2[2] = {(double*)(ws + Lo.scratch), (double*)(ws + Lo.scratch) + l};
+  double* C2[2] = {P2[1] + l, P2[1] + 2 * l};
+  double* hv0 = C2[1] + l;
   hipLaunchKernelGGL(sym_copy_kernel, dim3(ceil_div((long)l * l, 256) < 4096 ? ceil_div((long)l * l, 256) : 4096),
                      dim3(256), 0, st, A, l, ld, W);
-  for (int j = 0; j + 2 < l; ++j) {
-    const int n2 = l - j - 1;
-    const int waves = HH_THREADS / 64;
-    int g = (n2 + waves - 1) / waves;       // one wave per row
+  const int waves = HH_THREADS / 64;
+  {
+    int g = (l - 1 + waves - 1) / waves;    // one wave per row
     if (g > 256) g = 256;
-    hipLaunchKernelGGL(hh_mv_kernel, dim3(g), dim3(HH_THREADS), 0, st, (const double*)W, l, j, P);
-    hipLaunchKernelGGL(hh_up_kernel, dim3(g), dim3(HH_THREADS), 0, st, W, l, j, (const double*)P, d, e, tau, hv0);
+    hipLaunchKernelGGL(hh_start_kernel, dim3(g), dim3(HH_THREADS), 0, st, (const double*)W, l, C2[0], P2[0]);
   }
+  for (int j = 0; j + 2 < l; ++j) {
+    const int rows = l - j - 2;             // rows updated (the first row of the block is not)
+    int g = (rows + waves - 1) / waves;
+    if (g > 256) g = 256;
+    hipLaunchKernelGGL(hh_step_kernel, dim3(g), dim3(HH_THREADS), 0, st, W, l, j, (const double*)C2[j & 1],
+                       C2[(j + 1) & 1], (const double*)P2[j & 1], P2[(j + 1) & 1], d, e, tau, hv0);
+  }
+  hipLaunchKernelGGL(hh_tail_kernel, dim3(1), dim3(64), 0, st, W, l, (const double*)C2[(l - 2) & 1]);
   hipLaunchKernelGGL(hh_rows_kernel, dim3(ceil_div((long)l * l, 256) < 4096 ? ceil_div((long)l * l, 256) : 4096),
                      dim3(256), 0, st, W, l, (const double*)hv0);
   hipLaunchKernelGGL(syevx_vectors_kernel, dim3(1), dim3(EIG_THREADS), 0, st, l, nev, evals, ws);
@@ -280,55 +351,85 @@ extern "C" int dftb_syevx(const double* A, int l, int ld, int nev, double* evals
 // converges to Y = (A/c)^{1/2}, Z = (A/c)^{-1/2} for SPD A with spectrum in (0, c].
 // Replaces scipy.linalg.sqrtm(Mm) of Solver.py:762.
 // ---------------------------------------------------------------------------
-__global__ void ns_init_kernel(const double* A, int l, int ld, double* Y, double* Z, const double* scale) {
+#define NS_MAXIT 100
+#define RED_BLOCKS 256
+
+// atomic maximum of non-negative doubles through their bit patterns (which order
+// like the values); the target starts at +0.0
+__device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
+  atomicMax((unsigned long long*)p, (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ double blk_max(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  double m = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, sh[i]);
+  return m;
+}
+
+static int red_blocks(long n) {
+  long g = (n + 255) / 256;
+  return (int)(g < 1 ? 1 : (g > RED_BLOCKS ? RED_BLOCKS : g));
+}
+
+// Y0 = sym(A) / c, Z0 = I; errs[0 .. NS_MAXIT) = 0
+__global__ void ns_init_kernel(const double* A, int l, int ld, double* Y, double* Z, const double* scale, double* errs) {
   const double c = scale[0];
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (long)l * l; i += (long)gridDim.x * blockDim.x) {
+  const long g0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g0 < NS_MAXIT) errs[g0] = 0.0;
+  for (long i = g0; i < (long)l * l; i += (long)gridDim.x * blockDim.x) {
     int r = (int)(i / l), q = (int)(i % l);
     Y[i] = 0.5 * (A[(long)r * ld + q] + A[(long)q * ld + r]) / c;
     Z[i] = (r == q) ? 1.0 : 0.0;
   }
 }
 
-// scale[0] = Frobenius norm of the symmetric part (>= largest eigenvalue)
-__global__ __launch_bounds__(256) void fro_kernel(const double* A, int l, int ld, double* scale) {
+// part[b] = sum of squares of the workgroup's elements; the final pass adds the
+// parts in order: scale[0] = Frobenius norm (>= the largest eigenvalue of the
+// symmetric part)
+__global__ __launch_bounds__(256) void fro_part_kernel(const double* A, int l, int ld, double* part) {
   __shared__ double sh[4];
   double s = 0.0;
-  for (long i = threadIdx.x; i < (long)l * l; i += blockDim.x) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (long)l * l; i += (long)gridDim.x * blockDim.x) {
     int r = (int)(i / l), q = (int)(i % l);
     double a = A[(long)r * ld + q];
     s += a * a;
   }
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) scale[0] = fmax(sqrt(sh[0] + sh[1] + sh[2] + sh[3]), 1e-300);
+  s = blk_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// T = (3I - P)/2 ; err[0] = max |P - I|
-__global__ __launch_bounds__(1024) void ns_t_kernel(const double* P, int l, double* T, double* err) {
-  __shared__ double sh[16];
+__global__ __launch_bounds__(64) void sum_parts_kernel(const double* part, int n, double* out, int do_sqrt) {
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += part[i];
+    out[0] = do_sqrt ? fmax(sqrt(s), 1e-300) : s;
+  }
+}
+
+// T = (3I - P)/2 ; err[0] = max |P - I| (atomic, err zeroed by ns_init)
+__global__ __launch_bounds__(256) void ns_t_kernel(const double* P, int l, double* T, double* err) {
+  __shared__ double sh[4];
   double m = 0.0;
-  for (long i = threadIdx.x; i < (long)l * l; i += blockDim.x) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (long)l * l; i += (long)gridDim.x * blockDim.x) {
     int r = (int)(i / l), q = (int)(i % l);
     double id = (r == q) ? 1.0 : 0.0;
     double p = P[i];
     m = fmax(m, fabs(p - id));
     T[i] = 0.5 * (3.0 * id - p);
   }
-  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double mm = 0.0;
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) mm = fmax(mm, sh[i]);
-    err[0] = mm;
-  }
+  m = blk_max(m, sh);
+  if (threadIdx.x == 0) atomic_max_nonneg(err, m);
 }
 
 // out = X * sqrt(c) (sqrt) or X / sqrt(c) (inverse sqrt)
-__global__ void ns_finish_kernel(double* X, long n, const double* scale, int inverse) {
+__global__ void ns_finish_kernel(const double* X, double* out, long n, const double* scale, int inverse) {
   const double f = inverse ? 1.0 / sqrt(scale[0]) : sqrt(scale[0]);
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) X[i] *= f;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) out[i] = X[i] * f;
 }
 
 static GemmArgs small_gemm(const double* A, int lda, bool at, const double* B, int ldb, double* C, int ldc, int M,
@@ -341,34 +442,42 @@ static GemmArgs small_gemm(const double* A, int lda, bool at, const double* B, i
   return g;
 }
 
-static size_t ns_workspace(int l) { return al256((size_t)l * l * 8) * 5 + 256 * 2; }
+static size_t ns_workspace(int l) { return al256((size_t)l * l * 8) * 5 + al256((size_t)(1 + NS_MAXIT) * 8); }
 
 // Y <- A^{1/2}, Z <- A^{-1/2}; iters[0] = iterations used; returns DFTB_ERR_NOT_SPD if not converged.
 static int sqrtm_ns(const double* A, int l, int ld, double* Y, double* Z, char* ws, hipStream_t st, int* iters_out,
                     double* err_out) {
+  const size_t blk = al256((size_t)l * l * 8);
   double* T = (double*)ws;
-  double* P = (double*)(ws + al256((size_t)l * l * 8));
-  double* Y2 = (double*)(ws + 2 * al256((size_t)l * l * 8));
-  double* Z2 = (double*)(ws + 3 * al256((size_t)l * l * 8));
-  double* scal = (double*)(ws + 5 * al256((size_t)l * l * 8));
-  double* err = scal + 1;
+  double* P = (double*)(ws + blk);
+  double* Y2 = (double*)(ws + 2 * blk);
+  double* Z2 = (double*)(ws + 3 * blk);
+  double* scal = (double*)(ws + 5 * blk);
+  double* errs = scal + 1;                     // max |Z Y - I| of every iteration
+  const long n = (long)l * l;
+  const int g = red_blocks(n);
   int rc;
-  hipLaunchKernelGGL(fro_kernel, dim3(1), dim3(256), 0, st, A, l, ld, scal);
-  hipLaunchKernelGGL(ns_init_kernel, dim3((l * l + 255) / 256), dim3(256), 0, st, A, l, ld, Y, Z, scal);
+  hipLaunchKernelGGL(fro_part_kernel, dim3(g), dim3(256), 0, st, A, l, ld, T);
+  hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(64), 0, st, (const double*)T, g, scal, 1);
+  hipLaunchKernelGGL(ns_init_kernel, dim3((l * l + 255) / 256 > 1 ? (l * l + 255) / 256 : 1), dim3(256), 0, st, A, l,
+                     ld, Y, Z, (const double*)scal, errs);
   DFTB_CHECK_LAUNCH();
+  // the iterates alternate between the outputs and Y2 / Z2
+  double *y = Y, *z = Z, *y2 = Y2, *z2 = Z2;
   double herr = 1.0;
   int it = 0;
-  const int maxit = 100;
-  for (it = 0; it < maxit; ++it) {
-    if ((rc = gemm_launch(small_gemm(Z, l, false, Y, l, P, l, l, l, l), st))) return rc;
-    hipLaunchKernelGGL(ns_t_kernel, dim3(1), dim3(1024), 0, st, P, l, T, err);
+  for (it = 0; it < NS_MAXIT; ++it) {
+    if ((rc = gemm_launch(small_gemm(z, l, false, y, l, P, l, l, l, l), st))) return rc;
+    hipLaunchKernelGGL(ns_t_kernel, dim3(g), dim3(256), 0, st, (const double*)P, l, T, errs + it);
     DFTB_CHECK_LAUNCH();
-    if ((rc = gemm_launch(small_gemm(Y, l, false, T, l, Y2, l, l, l, l), st))) return rc;
-    if ((rc = gemm_launch(small_gemm(T, l, false, Z, l, Z2, l, l, l, l), st))) return rc;
-    if (hipMemcpyAsync(Y, Y2, (size_t)l * l * 8, hipMemcpyDeviceToDevice, st) != hipSuccess) return DFTB_ERR_LAUNCH;
-    if (hipMemcpyAsync(Z, Z2, (size_t)l * l * 8, hipMemcpyDeviceToDevice, st) != hipSuccess) return DFTB_ERR_LAUNCH;
+    if ((rc = gemm_launch(small_gemm(y, l, false, T, l, y2, l, l, l, l), st))) return rc;
+    if ((rc = gemm_launch(small_gemm(T, l, false, z, l, z2, l, l, l, l), st))) return rc;
+    double* t;
+    t = y; y = y2; y2 = t;
+    t = z; z = z2; z2 = t;
     if (it % 4 == 3) {
-      if (hipMemcpyAsync(&herr, err, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) return DFTB_ERR_LAUNCH;
+      if (hipMemcpyAsync(&herr, errs + it, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
+        return DFTB_ERR_LAUNCH;
       if (hipStreamSynchronize(st) != hipSuccess) return DFTB_ERR_LAUNCH;
       if (!(herr == herr) || herr > 1e6) break;  // diverged: not SPD
       if (herr < 1e-14 * (l > 16 ? l : 16)) break;
@@ -376,8 +485,9 @@ static int sqrtm_ns(const double* A, int l, int ld, double* Y, double* Z, char* 
   }
   if (iters_out) *iters_out = it + 1;
   if (err_out) *err_out = herr;
-  hipLaunchKernelGGL(ns_finish_kernel, dim3((l * l + 255) / 256), dim3(256), 0, st, Y, (long)l * l, scal, 0);
-  hipLaunchKernelGGL(ns_finish_kernel, dim3((l * l + 255) / 256), dim3(256), 0, st, Z, (long)l * l, scal, 1);
+  const int gf = (l * l + 255) / 256 > 1 ? (l * l + 255) / 256 : 1;
+  hipLaunchKernelGGL(ns_finish_kernel, dim3(gf), dim3(256), 0, st, (const double*)y, Y, n, (const double*)scal, 0);
+  hipLaunchKernelGGL(ns_finish_kernel, dim3(gf), dim3(256), 0, st, (const double*)z, Z, n, (const double*)scal, 1);
   DFTB_CHECK_LAUNCH();
   if (!(herr < 1e-10)) return DFTB_ERR_NOT_SPD;
   return DFTB_OK;
@@ -392,18 +502,16 @@ extern "C" int dftb_sqrtm(const double* A, int l, int ld, double* Asq, double* A
   return sqrtm_ns(A, l, ld, Asq, Ainvsq, (char*)workspace, (hipStream_t)stream, iters, err);
 }
 
-// err = sum |Mh - Mh^T|  (Solver.py:766-770)
-__global__ __launch_bounds__(256) void herm_err_kernel(const double* M, int l, double* diag) {
+// err = sum |Mh - Mh^T|  (Solver.py:766-770): per-workgroup parts, added in order
+__global__ __launch_bounds__(256) void herm_part_kernel(const double* M, int l, double* part) {
   __shared__ double sh[4];
   double s = 0.0;
-  for (long i = threadIdx.x; i < (long)l * l; i += blockDim.x) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (long)l * l; i += (long)gridDim.x * blockDim.x) {
     int r = (int)(i / l), c = (int)(i % l);
     s += fabs(M[(long)r * l + c] - M[(long)c * l + r]);
   }
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) diag[0] = sh[0] + sh[1] + sh[2] + sh[3];
+  s = blk_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 // w = sqrt(w2) (flag non-positive w2); scales: sR[n] = 1/sqrt(w), sL[n] = 1/w
@@ -502,8 +610,13 @@ extern "C" int dftb_subspace_nonhermitian(const double* Mp, const double* Mm, in
   // Mh = Mmsq Mp Mmsq, hermiticity check
   if ((rc = gemm_launch(small_gemm(Mp, ld, false, Sq, l, X, l, l, l, l), st))) return rc;
   if ((rc = gemm_launch(small_gemm(Sq, l, false, X, l, H, l, l, l, l), st))) return rc;
-  hipLaunchKernelGGL(herm_err_kernel, dim3(1), dim3(256), 0, st, H, l, diag);
-  DFTB_CHECK_LAUNCH();
+  {
+    const int g = red_blocks((long)l * l);
+    double* part = (double*)(ws + S.eig);     // free between the square root and the eigensolver
+    hipLaunchKernelGGL(herm_part_kernel, dim3(g), dim3(256), 0, st, (const double*)H, l, part);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(64), 0, st, (const double*)part, g, diag, 0);
+    DFTB_CHECK_LAUNCH();
+  }
   if ((rc = dftb_syevx(H, l, l, nev, w2, Tb, ws + S.eig, eig_layout(l, nev).total, stream))) return rc;
   hipLaunchKernelGGL(roots_kernel, dim3((nev + 63) / 64), dim3(64), 0, st, w2, nev, w, sR, sL, info);
   DFTB_CHECK_LAUNCH();
@@ -532,11 +645,16 @@ extern "C" int dftb_subspace_hermitian(const double* H, int l, int ld, int nev, 
   return dftb_syevx(H, l, ld, nev, w2, T, (char*)workspace + S.eig, eig_layout(l, nev).total, stream);
 }
 
-// Replace the leading l x l block of M by its symmetric part; asym[0] = max |M - M^T| / max |M|.
-__global__ __launch_bounds__(1024) void symmetrize_kernel(double* M, int l, int ld, double* asym) {
-  __shared__ double sh[16], sm[16];
+// Replace the leading l x l block of M by its symmetric part; asym[0] = max |M - M^T| / max |M|
+// (asym[1], asym[2]: the two maxima, accumulated atomically from zero).
+__global__ void symmetrize_zero_kernel(double* asym) {
+  if (threadIdx.x < 3) asym[threadIdx.x] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void symmetrize_kernel(double* M, int l, int ld, double* asym) {
+  __shared__ double sh[4];
   double a = 0.0, mx = 0.0;
-  for (long i = threadIdx.x; i < (long)l * l; i += blockDim.x) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (long)l * l; i += (long)gridDim.x * blockDim.x) {
     int r = (int)(i / l), c = (int)(i % l);
     if (c > r) {
       double x = M[(long)r * ld + c], y = M[(long)c * ld + r];
@@ -549,28 +667,24 @@ __global__ __launch_bounds__(1024) void symmetrize_kernel(double* M, int l, int 
       mx = fmax(mx, fabs(M[(long)r * ld + c]));
     }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    a = fmax(a, __shfl_xor(a, o, 64));
-    mx = fmax(mx, __shfl_xor(mx, o, 64));
-  }
-  if ((threadIdx.x & 63) == 0) {
-    sh[threadIdx.x >> 6] = a;
-    sm[threadIdx.x >> 6] = mx;
-  }
-  __syncthreads();
+  a = blk_max(a, sh);
+  mx = blk_max(mx, sh);
   if (threadIdx.x == 0) {
-    double A = 0.0, X = 0.0;
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
-      A = fmax(A, sh[i]);
-      X = fmax(X, sm[i]);
-    }
-    asym[0] = X > 0.0 ? A / X : 0.0;
+    atomic_max_nonneg(asym + 1, a);
+    atomic_max_nonneg(asym + 2, mx);
   }
+}
+
+__global__ void symmetrize_ratio_kernel(double* asym) {
+  if (threadIdx.x == 0) asym[0] = asym[2] > 0.0 ? asym[1] / asym[2] : 0.0;
 }
 
 extern "C" int dftb_symmetrize(double* M, int l, int ld, double* asym, void* stream) {
   if (l <= 0 || ld < l || !M || !asym) return DFTB_ERR_ARG;
-  hipLaunchKernelGGL(symmetrize_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, M, l, ld, asym);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(symmetrize_zero_kernel, dim3(1), dim3(64), 0, st, asym);
+  hipLaunchKernelGGL(symmetrize_kernel, dim3(red_blocks((long)l * l)), dim3(256), 0, st, M, l, ld, asym);
+  hipLaunchKernelGGL(symmetrize_ratio_kernel, dim3(1), dim3(64), 0, st, asym);
   DFTB_CHECK_LAUNCH();
   return DFTB_OK;
 }

@@ -665,7 +665,7 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
     new0 = 0
     info = torch.zeros(2, dtype=torch.int32, device=dev)
     diag = torch.zeros(2, dtype=torch.float64, device=dev)
-    asym = torch.zeros(1, dtype=torch.float64, device=dev)
+    asym = torch.zeros(3, dtype=torch.float64, device=dev)
     wdev = torch.zeros(k, dtype=torch.float64, device=dev)
     Tb = torch.zeros((cap, k), dtype=torch.float64, device=dev)
     Rb = torch.zeros((cap, k), dtype=torch.float64, device=dev)
@@ -1086,7 +1086,7 @@ def zvector_gpu(op, B, X0=None, maxiter=1000, conv=1.0e-14, precision=0, verbose
         gram(bs, l, P, l, N, Ab, ws)
         Bb = torch.empty((l, k), dtype=torch.float64, device=dev)
         gram(bs, l, Bd, k, N, Bb, ws)
-        asym = torch.zeros(1, dtype=torch.float64, device=dev)
+        asym = torch.zeros(3, dtype=torch.float64, device=dev)
         call("dftb_symmetrize", _p(Ab), l, l, _p(asym), st)
         # Xb = V diag(1/lambda) V^T Bb with Ab = V diag(lambda) V^T (GPU eigensolver)
         lam = torch.empty(l, dtype=torch.float64, device=dev)

@@ -235,7 +235,8 @@ size_t dftb_subspace_workspace(int l, int nev);
 int dftb_subspace_nonhermitian(const double* Mp, const double* Mm, int l, int ld, int nev,
                                double* w, double* Tb, double* Rb, double* Lb, int* info, double* diag,
                                void* workspace, size_t ws_bytes, void* stream);
-/* M[:l,:l] <- (M + M^T)/2; asym[0] = max|M - M^T| / max|M|.  Used for the
+/* M[:l,:l] <- (M + M^T)/2; asym[0] = max|M - M^T| / max|M| (asym holds 3 doubles:
+ * asym[1] and asym[2] receive the two maxima).  Used for the
  * subspace matrices in the mixed-precision modes, where the products carry
  * FP32-level rounding and the reference's 1e-10 Hermiticity test
  * (Solver.py:766-770) applies to FP64 only. */

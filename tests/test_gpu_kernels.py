@@ -677,6 +677,78 @@ def test_orthonormalize(rng):
         assert np.linalg.norm(res) < 1e-9 * np.linalg.norm(row)
 
 
+def test_orthonormalize_without_basis(rng):
+    """l = 0 (a restart from given vectors): Cholesky-QR of the block alone."""
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _p, _stream, Workspace
+    n, m = 9001, 12
+    Qh = rng.standard_normal((m, n))
+    Qh[4] = 2.0 * Qh[1] - Qh[0]
+    Qd = T(Qh)
+    keep = torch.empty(m, dtype=torch.int32, device=_dev())
+    w, wn = Workspace(_dev()).get(_lib.load().dftb_orthonormalize_workspace(0, m, n))
+    _lib.call("dftb_orthonormalize", None, 0, _p(Qd), m, n, 1e-8, _p(keep), _p(w), wn, _stream())
+    k = keep.cpu().numpy()
+    assert list(k) == [1, 1, 1, 1, 0] + [1] * (m - 5)
+    Qr = Qd.cpu().numpy()[k == 1]
+    np.testing.assert_allclose(Qr @ Qr.T, np.eye(m - 1), atol=1e-12)
+    for row in Qh[k == 1]:
+        res = row - Qr.T @ (Qr @ row)
+        assert np.linalg.norm(res) < 1e-9 * np.linalg.norm(row)
+
+
+@pytest.mark.parametrize("m", [40, 60])
+def test_orthonormalize_block(m, rng):
+    """Blocks of the Davidson's size (the LDS Cholesky-QR and the register
+    triangular update, m <= 48) and beyond (global-memory factor, GEMM update)."""
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _p, _stream, Workspace
+    n, l = 20011, 100
+    B, _ = np.linalg.qr(rng.standard_normal((n, l)))
+    Qh = rng.standard_normal((m, n))
+    Qh[7] = 0.5 * Qh[2] - Qh[5] + B[:, 9]      # dependent after the projection
+    Qh[m - 1] = B[:, 3] * 1e-3                  # inside span(B)
+    Bd = T(B.T.copy())
+    Qd = T(Qh)
+    keep = torch.empty(m, dtype=torch.int32, device=_dev())
+    w, wn = Workspace(_dev()).get(_lib.load().dftb_orthonormalize_workspace(l, m, n))
+    _lib.call("dftb_orthonormalize", _p(Bd), l, _p(Qd), m, n, 1e-8, _p(keep), _p(w), wn, _stream())
+    k = keep.cpu().numpy()
+    want = np.ones(m, dtype=int)
+    want[7] = want[m - 1] = 0
+    assert list(k) == list(want)
+    Qr = Qd.cpu().numpy()[k == 1]
+    np.testing.assert_allclose(Qr @ Qr.T, np.eye(m - 2), atol=1e-12)
+    np.testing.assert_allclose(Qr @ B, 0, atol=1e-12)
+    assert np.abs(Qd.cpu().numpy()[k == 0]).max() == 0.0
+    basis = np.vstack([B.T, Qr])
+    for row in Qh[k == 1]:
+        res = row - basis.T @ (basis @ row)
+        assert np.linalg.norm(res) < 1e-9 * np.linalg.norm(row)
+
+
+def test_symmetrize(rng):
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _p, _stream
+    l, ld = 300, 310
+    A = rng.standard_normal((l, l))
+    S = A + A.T
+    E = 1e-6 * rng.standard_normal((l, l))
+    M = np.zeros((l, ld))
+    M[:, :l] = S + E
+    M[:, l:] = 7.0                                 # beyond the leading block: untouched
+    Md = T(M)
+    asym = torch.full((3,), 5.0, dtype=torch.float64, device=_dev())
+    _lib.call("dftb_symmetrize", _p(Md), l, ld, _p(asym), _stream())
+    out = Md.cpu().numpy()
+    want = 0.5 * ((S + E) + (S + E).T)
+    np.testing.assert_array_equal(out[:, :l], out[:, :l].T)
+    np.testing.assert_allclose(out[:, :l], want, rtol=0, atol=1e-15 * np.abs(S).max())
+    assert (out[:, l:] == 7.0).all()
+    ref = np.abs((S + E) - (S + E).T).max() / np.abs(S + E).max()
+    assert abs(asym.cpu().numpy()[0] - ref) <= 1e-15 * ref
+
+
 def test_autotune_ao_tiles_are_used(tmp_path):
     """The project autotuner times the BF16x3 AO chain's pipelined tiles on full
     block matvecs, stores its pick in op.tuning (passed to every later matvec)
