@@ -258,25 +258,19 @@ static void cholqr_launch(const double* G, int m, const double* ref, double tol2
 }
 
 // dst[i][e] = sum_{k <= i} Linv[i][k] src[k][e] for a lower-triangular m x m Linv
-// (m <= MP): one column e per thread with src[0..m)[e] in registers and Linv in
-// LDS (zero-padded to MP x MP, uniform addresses: broadcast reads).  Streams src
-// once and writes dst once.
+// (m <= MP): one column e per thread with src[0..m)[e] in registers; the entries
+// of Linv have workgroup-uniform addresses (scalar loads, no LDS traffic).  Streams
+// src once and writes dst once.
 template <int MP>
 __global__ __launch_bounds__(256) void tri_apply_kernel(const double* __restrict__ Linv, int m,
                                                         const double* __restrict__ src, double* __restrict__ dst,
                                                         long n) {
-  __shared__ double sL[MP * MP];
-  for (int i = threadIdx.x; i < MP * MP; i += blockDim.x) {
-    const int r = i / MP, c = i % MP;
-    sL[i] = (r < m && c < m) ? Linv[r * m + c] : 0.0;
-  }
-  __syncthreads();
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
     double x[MP];
 #pragma unroll
     for (int k = 0; k < MP; ++k) x[k] = (k < m) ? src[(long)k * n + e] : 0.0;
     for (int i = 0; i < m; ++i) {
-      const double* li = sL + i * MP;
+      const double* li = Linv + i * m;
       double s = 0.0;
 #pragma unroll
       for (int k = 0; k < MP; ++k)
