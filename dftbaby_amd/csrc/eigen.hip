@@ -111,19 +111,16 @@ __global__ __launch_bounds__(HH_THREADS) void hh_start_kernel(const double* __re
 
 // col[0] = updated (j, j), col[1 + r] = updated (j + 1 + r, j); P[r] = t_j (A22 v_j)_r
 // (r < n2).  Writes coln (updated column j + 1 from its diagonal down) and Pn (n2 - 1).
-__global__ __launch_bounds__(HH_THREADS) void hh_step_kernel(double* __restrict__ W, int l, int j,
-                                                             const double* __restrict__ col, double* __restrict__ coln,
-                                                             const double* __restrict__ P, double* __restrict__ Pn,
-                                                             double* __restrict__ d, double* __restrict__ e,
-                                                             double* __restrict__ tau, double* __restrict__ hv0) {
-  __shared__ double sv[EIG_MAXL];   // v_j
-  __shared__ double sw[EIG_MAXL];   // w_j
-  __shared__ double sn[EIG_MAXL];   // v_{j+1}
-  __shared__ double sh[HH_THREADS / 64];
+// Workgroup bid of nblk takes rows 1 + (bid * waves + wave) + k * nblk * waves.
+__device__ void hh_step_body(double* __restrict__ W, int l, int j, const double* __restrict__ col,
+                             double* __restrict__ coln, const double* __restrict__ P, double* __restrict__ Pn,
+                             double* __restrict__ d, double* __restrict__ e, double* __restrict__ tau,
+                             double* __restrict__ hv0, int bid, int nblk, double* sv, double* sw, double* sn,
+                             double* sh) {
   const int n2 = l - j - 1, tid = threadIdx.x;
   double beta;
   const double t = hh_vec(col + 1, n2, sv, sh, &beta);
-  if (blockIdx.x == 0 && tid == 0) {
+  if (bid == 0 && tid == 0) {
     tau[j] = t;
     e[j] = beta;
     d[j] = col[0];
@@ -153,7 +150,7 @@ __global__ __launch_bounds__(HH_THREADS) void hh_step_kernel(double* __restrict_
   double part = 0.0;
   for (int r = tid; r < n2; r += blockDim.x) {
     const double a = rowp[r] - sym_upd(v0, sw[r], w0, sv[r]);
-    if (blockIdx.x == 0) coln[r] = a;
+    if (bid == 0) coln[r] = a;
     if (r >= 1) {
       sn[r - 1] = a;
       part += a * a;
@@ -172,7 +169,7 @@ __global__ __launch_bounds__(HH_THREADS) void hh_step_kernel(double* __restrict_
   }
   __syncthreads();
   const int lane = tid & 63, nw = blockDim.x >> 6;
-  for (int r = 1 + blockIdx.x * nw + (tid >> 6); r < n2; r += gridDim.x * nw) {
+  for (int r = 1 + bid * nw + (tid >> 6); r < n2; r += nblk * nw) {
     double* row = W + (long)(j + 1 + r) * l + (j + 1);
     const double vr = sv[r], wr = sw[r];
     double s = 0.0;
@@ -187,6 +184,18 @@ __global__ __launch_bounds__(HH_THREADS) void hh_step_kernel(double* __restrict_
       W[(long)(j + 1 + r) * l + j] = vr;   // v_j below its first entry (0 when t = 0)
     }
   }
+}
+
+__global__ __launch_bounds__(HH_THREADS) void hh_step_kernel(double* __restrict__ W, int l, int j,
+                                                             const double* __restrict__ col, double* __restrict__ coln,
+                                                             const double* __restrict__ P, double* __restrict__ Pn,
+                                                             double* __restrict__ d, double* __restrict__ e,
+                                                             double* __restrict__ tau, double* __restrict__ hv0) {
+  __shared__ double sv[EIG_MAXL];   // v_j
+  __shared__ double sw[EIG_MAXL];   // w_j
+  __shared__ double sn[EIG_MAXL];   // v_{j+1}
+  __shared__ double sh[HH_THREADS / 64];
+  hh_step_body(W, l, j, col, coln, P, Pn, d, e, tau, hv0, blockIdx.x, gridDim.x, sv, sw, sn, sh);
 }
 
 // The last 2 x 2 block: the updated (l-2, l-2) and (l-1, l-2) from col.
@@ -388,26 +397,35 @@ __global__ void ns_init_kernel(const double* A, int l, int ld, double* Y, double
   }
 }
 
-// part[b] = sum of squares of the workgroup's elements; the final pass adds the
-// parts in order: scale[0] = Frobenius norm (>= the largest eigenvalue of the
-// symmetric part)
-__global__ __launch_bounds__(256) void fro_part_kernel(const double* A, int l, int ld, double* part) {
+// part[b] = largest absolute row sum of the symmetric part over the workgroup's rows
+// (one wave per row); the maximum over the parts bounds the largest eigenvalue
+// (Gershgorin) and scales the spectrum into (0, 1].
+__global__ __launch_bounds__(256) void gersh_part_kernel(const double* A, int l, int ld, double* part) {
   __shared__ double sh[4];
-  double s = 0.0;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (long)l * l; i += (long)gridDim.x * blockDim.x) {
-    int r = (int)(i / l), q = (int)(i % l);
-    double a = A[(long)r * ld + q];
-    s += a * a;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double m = 0.0;
+  for (int i = blockIdx.x * 4 + wv; i < l; i += gridDim.x * 4) {
+    double s = 0.0;
+    for (int j = lane; j < l; j += 64) s += fabs(0.5 * (A[(long)i * ld + j] + A[(long)j * ld + i]));
+    m = fmax(m, wave_sum(s));
   }
-  s = blk_sum(s, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  m = blk_max(m, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
-__global__ __launch_bounds__(64) void sum_parts_kernel(const double* part, int n, double* out, int do_sqrt) {
+__global__ __launch_bounds__(64) void max_parts_kernel(const double* part, int n, double* out) {
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int i = 0; i < n; ++i) m = fmax(m, part[i]);
+    out[0] = fmax(m, 1e-300);
+  }
+}
+
+__global__ __launch_bounds__(64) void sum_parts_kernel(const double* part, int n, double* out) {
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int i = 0; i < n; ++i) s += part[i];
-    out[0] = do_sqrt ? fmax(sqrt(s), 1e-300) : s;
+    out[0] = s;
   }
 }
 
@@ -457,8 +475,9 @@ static int sqrtm_ns(const double* A, int l, int ld, double* Y, double* Z, char* 
   const long n = (long)l * l;
   const int g = red_blocks(n);
   int rc;
-  hipLaunchKernelGGL(fro_part_kernel, dim3(g), dim3(256), 0, st, A, l, ld, T);
-  hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(64), 0, st, (const double*)T, g, scal, 1);
+  const int gr = (l + 3) / 4 < RED_BLOCKS ? (l + 3) / 4 : RED_BLOCKS;
+  hipLaunchKernelGGL(gersh_part_kernel, dim3(gr), dim3(256), 0, st, A, l, ld, T);
+  hipLaunchKernelGGL(max_parts_kernel, dim3(1), dim3(64), 0, st, (const double*)T, gr, scal);
   hipLaunchKernelGGL(ns_init_kernel, dim3((l * l + 255) / 256 > 1 ? (l * l + 255) / 256 : 1), dim3(256), 0, st, A, l,
                      ld, Y, Z, (const double*)scal, errs);
   DFTB_CHECK_LAUNCH();
@@ -614,7 +633,7 @@ extern "C" int dftb_subspace_nonhermitian(const double* Mp, const double* Mm, in
     const int g = red_blocks((long)l * l);
     double* part = (double*)(ws + S.eig);     // free between the square root and the eigensolver
     hipLaunchKernelGGL(herm_part_kernel, dim3(g), dim3(256), 0, st, (const double*)H, l, part);
-    hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(64), 0, st, (const double*)part, g, diag, 0);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(64), 0, st, (const double*)part, g, diag);
     DFTB_CHECK_LAUNCH();
   }
   if ((rc = dftb_syevx(H, l, l, nev, w2, Tb, ws + S.eig, eig_layout(l, nev).total, stream))) return rc;

@@ -294,10 +294,15 @@ int gemm_launch(const GemmArgs& p0, hipStream_t st, bool timed) {
     else if (small && p.N <= 32 && p.M > 48) launch_fp<double, 64, 32, 16, 4>(p, st);
     else if (small && p.N <= 48 && p.M > 48) launch_fp<double, 64, 48, 16, 4>(p, st);
     else if (small && p.N <= 80 && p.M > 80) launch_fp<double, 64, 80, 16, 4>(p, st);   // both Gram halves of a block
+    else if (small && p.M <= 48 && p.N <= 48 && p.M > 32) launch_fp<double, 48, 64, 16, 1>(p, st);  // a block's own Gram
     else if (small) launch_fp<double, 64, 64, 16>(p, st);
     else launch_fp<double, 128, 128, 16>(p, st);
   } else {
-    if (small) launch_fp<float, 64, 64, 32>(p, st);
+    // the Coulomb projection / expansion of a vector block: M = block size (<= 48 rows
+    // per tile instead of 64 cuts the padded MFMA work by a quarter)
+    if (small && p.M <= 48 && p.M > 32 && p.N >= 4096) launch_fp<float, 48, 128, 32, 1>(p, st);
+    else if (small && p.M <= 48 && p.M > 32) launch_fp<float, 48, 64, 32, 1>(p, st);
+    else if (small) launch_fp<float, 64, 64, 32>(p, st);
     else launch_fp<float, 128, 128, 32>(p, st);
   }
   const int cls = p.in_type == GEMM_IN_F64

@@ -581,7 +581,7 @@ def test_hermitian_matvec(rng):
 # ---------------------------------------------------------------------------
 # (c) subspace kernels
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("l,nev", [(1, 1), (2, 2), (9, 4), (60, 20), (257, 30)])
+@pytest.mark.parametrize("l,nev", [(1, 1), (2, 2), (9, 4), (60, 20), (180, 12), (257, 30)])
 def test_syevx(l, nev, rng):
     from dftbaby_amd import _lib
     from dftbaby_amd.engine import _p, _stream, Workspace
