@@ -102,6 +102,8 @@ _SIGS = {
     "dftb_orthonormalize": (_i, [_vp, _i, _vp, _i, _l, _d, _vp, _vp, _sz, _vp]),
     "dftb_subspace_workspace": (_sz, [_i, _i]),
     "dftb_subspace_nonhermitian": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "dftb_subspace_nonhermitian_rl": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "dftb_subspace_tb": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
     "dftb_sqrtm_workspace": (_sz, [_i]),
     "dftb_sqrtm": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "dftb_subspace_hermitian": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _sz, _vp]),

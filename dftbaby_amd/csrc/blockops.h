@@ -391,42 +391,54 @@ __device__ __noinline__ void syevx_tridiag_vectors(int l, int nev, double* __res
     }
   }
   __syncthreads();
-  // large clusters: simultaneous inverse iteration.  Every sweep solves all members
-  // at once (one thread each, factors and iterates in the scratch, L2-resident),
-  // then orthonormalises the iterates in member order (classical Gram-Schmidt
-  // twice against the members before, one wave per coefficient), so that
-  // degenerate members span their eigenspace with orthonormal vectors.
+  // large clusters, cooperatively.  The member's factors and iterate are staged in
+  // the LDS buffers (free once the factors exist) when 4 l doubles fit, so the
+  // serial solve runs on LDS; the Gram-Schmidt coefficients take one wave each.
   const int lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  const bool stage = 4 * l <= cap;
   for (int m0 = 0; m0 < nev;) {
     int m1 = m0 + 1;
     while (m1 < nev && evals[m1] - evals[m1 - 1] < ortol) ++m1;
     if (m1 - m0 > EIG_SMALL_CLUSTER) {
       for (int m = m0; m < m1; ++m) {
-        double* x = scr + (size_t)6 * l * m + 5 * l;
-        for (int i = tid; i < l; i += blockDim.x) x[i] = hash_unit((unsigned)m, (unsigned)i);
-      }
-      __syncthreads();
-      for (int it = 0; it < 3; ++it) {
-        for (int m = m0 + tid; m < m1; m += blockDim.x) {
-          double* base = scr + (size_t)6 * l * m;
-          tri_apply(l, (const double*)base, (const double*)(base + l), (const double*)(base + 2 * l),
-                    (const double*)(base + 3 * l), (const int*)(base + 4 * l), base + 5 * l);
+        double* base = scr + (size_t)6 * l * m;
+        const double *fu0 = base, *fu1 = base + l, *fu2 = base + 2 * l, *fml = base + 3 * l;
+        const int* fpv = (const int*)(base + 4 * l);
+        double* x = base + 5 * l;
+        if (stage) {
+          for (int i = tid; i < l; i += blockDim.x) {
+            sv[i] = base[i];
+            sv[l + i] = base[l + i];
+            sv[2 * l + i] = base[2 * l + i];
+            sv[3 * l + i] = base[3 * l + i];
+            ((int*)(sp + l))[i] = ((const int*)(base + 4 * l))[i];
+          }
+          fu0 = sv; fu1 = sv + l; fu2 = sv + 2 * l; fml = sv + 3 * l;
+          fpv = (const int*)(sp + l);
+          x = sp;
         }
+        for (int i = tid; i < l; i += blockDim.x) x[i] = hash_unit((unsigned)m, (unsigned)i);
         __syncthreads();
-        for (int m = m0; m < m1; ++m) {
-          double* x = scr + (size_t)6 * l * m + 5 * l;
+        for (int it = 0; it < 3; ++it) {
+          if (tid == 0) {
+            if (stage)
+              tri_apply(l, (const lds_f64*)fu0, (const lds_f64*)fu1, (const lds_f64*)fu2, (const lds_f64*)fml,
+                        (const lds_i32*)fpv, (lds_f64*)x);
+            else
+              tri_apply(l, fu0, fu1, fu2, fml, fpv, x);
+          }
+          __syncthreads();
           for (int pass = 0; pass < 2 && m > m0; ++pass) {
             for (int mm = m0 + wv; mm < m; mm += nw) {
-              const double* y = scr + (size_t)6 * l * mm + 5 * l;
               double s = 0.0;
-              for (int i = lane; i < l; i += 64) s += y[i] * x[i];
+              for (int i = lane; i < l; i += 64) s += Z[(long)i * nev + mm] * x[i];
               s = wave_sum(s);
               if (lane == 0) coef[mm] = s;
             }
             __syncthreads();
             for (int i = tid; i < l; i += blockDim.x) {
               double s = x[i];
-              for (int mm = m0; mm < m; ++mm) s -= coef[mm] * scr[(size_t)6 * l * mm + 5 * l + i];
+              for (int mm = m0; mm < m; ++mm) s -= coef[mm] * Z[(long)i * nev + mm];
               x[i] = s;
             }
             __syncthreads();
@@ -438,12 +450,9 @@ __device__ __noinline__ void syevx_tridiag_vectors(int l, int nev, double* __res
           for (int i = tid; i < l; i += blockDim.x) x[i] /= nrm;
           __syncthreads();
         }
-      }
-      for (int m = m0; m < m1; ++m) {
-        const double* x = scr + (size_t)6 * l * m + 5 * l;
         for (int i = tid; i < l; i += blockDim.x) Z[(long)i * nev + m] = x[i];
+        __syncthreads();
       }
-      __syncthreads();
     }
     m0 = m1;
   }

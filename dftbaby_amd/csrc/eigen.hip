@@ -20,6 +20,7 @@
 
 #define EIG_THREADS 512
 #define EIG_MAXL 4096
+#define EIG_MULTI_L 64   // from this size on: multi-workgroup reduction, then the one-workgroup tail
 
 extern "C" size_t dftb_syevx_workspace(int l, int nev) { return eig_layout(l, nev).total; }
 
@@ -301,7 +302,7 @@ extern "C" int dftb_syevx(const double* A, int l, int ld, int nev, double* evals
   if (ws_bytes < Lo.total) return DFTB_ERR_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
   char* ws = (char*)workspace;
-  if (l < 160) {
+  if (l < EIG_MULTI_L) {
     hipLaunchKernelGGL(syevx_kernel, dim3(1), dim3(EIG_THREADS), 0, st, A, l, ld, nev, evals, evecs, ws,
                        (int*)nullptr);
     DFTB_CHECK_LAUNCH();
@@ -649,6 +650,186 @@ extern "C" int dftb_subspace_nonhermitian(const double* Mp, const double* Mm, in
   // biorthonormality check Lb^T Rb = 1 (Solver.py:791-798)
   if ((rc = gemm_launch(small_gemm(Lb, nev, true, Rb, nev, G, nev, nev, nev, l), st))) return rc;
   hipLaunchKernelGGL(biortho_kernel, dim3(1), dim3(256), 0, st, G, nev, diag);
+  DFTB_CHECK_LAUNCH();
+  return DFTB_OK;
+}
+
+// ---- Cholesky route of the non-Hermitian subspace step ------------------------
+// With Mm = L L^T, H' = L^T Mp L is orthogonally similar to Mh = Mm^{1/2} Mp Mm^{1/2}
+// (Mh = Q^T H' Q with Q = L^{-1} Mm^{1/2}): the same roots w^2, and for
+// H' T' = T' diag(w^2) the eigenvectors of Mh are Tb = Q^T T', so that
+// Rb = Mm^{1/2} Tb / sqrt(w) = L T' / sqrt(w) and Lb = Mp Rb / w.  The iterations
+// need only the Cholesky factor (blocked: a one-workgroup factor of each 64 x 64
+// diagonal block with its inverse, the panel below as a GEMM with that inverse,
+// the trailing update as a GEMM); Tb itself, which only C = bs Tb after the last
+// iteration uses, is Mm^{-1/2} Rb diag(sqrt(w)) (dftb_subspace_tb).
+#define CHOL_NB 64
+
+__global__ void fill_zero_kernel(double* X, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) X[i] = 0.0;
+}
+
+// Factor the b x b diagonal block at (p, p) of W (lower triangle, trailing updates
+// applied): L[p.., p..] = its Cholesky factor (zeros above the diagonal), Linv = the
+// factor's inverse (b x b).  A non-positive pivot sets info (DFTB_ERR_NOT_SPD, pivot
+// index) and continues with a unit pivot, so that later launches see finite values.
+__global__ __launch_bounds__(256) void chol_diag_kernel(const double* __restrict__ W, double* __restrict__ L, int l,
+                                                        int p, int b, double* __restrict__ Linv, int* __restrict__ info) {
+  __shared__ double sA[CHOL_NB][CHOL_NB + 1];
+  __shared__ double sI[CHOL_NB][CHOL_NB + 1];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < b * b; i += blockDim.x) {
+    const int r = i / b, c = i % b;
+    sA[r][c] = (c <= r) ? W[(long)(p + r) * l + (p + c)] : 0.0;
+  }
+  __syncthreads();
+  for (int k = 0; k < b; ++k) {
+    if (tid == 0) {
+      double dk = sA[k][k];
+      if (!(dk > 0.0)) {
+        if (info[0] == 0) {
+          info[0] = DFTB_ERR_NOT_SPD;
+          info[1] = p + k;
+        }
+        dk = 1.0;
+      }
+      sA[k][k] = sqrt(dk);
+    }
+    __syncthreads();
+    const double pk = sA[k][k];
+    for (int i = k + 1 + tid; i < b; i += blockDim.x) sA[i][k] /= pk;
+    __syncthreads();
+    const int n = b - k - 1;
+    for (int t = tid; t < n * n; t += blockDim.x) {
+      const int i = k + 1 + t / n, j = k + 1 + t % n;
+      if (j <= i) sA[i][j] -= sA[i][k] * sA[j][k];
+    }
+    __syncthreads();
+  }
+  // inverse by forward substitution, one column per thread
+  for (int c = tid; c < b; c += blockDim.x) {
+    for (int i = 0; i < b; ++i) {
+      if (i < c) {
+        sI[i][c] = 0.0;
+        continue;
+      }
+      double s = (i == c) ? 1.0 : 0.0;
+      for (int k = c; k < i; ++k) s -= sA[i][k] * sI[k][c];
+      sI[i][c] = s / sA[i][i];
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < b * b; i += blockDim.x) {
+    const int r = i / b, c = i % b;
+    L[(long)(p + r) * l + (p + c)] = (c <= r) ? sA[r][c] : 0.0;
+    Linv[i] = sI[r][c];
+  }
+}
+
+// L (l x l, zero above the diagonal) with L L^T = sym(Mm); W: l x l scratch.
+static int cholesky_blocked(const double* Mm, int l, int ld, double* W, double* L, double* Linv, int* info,
+                            hipStream_t st) {
+  const int gcopy = ceil_div((long)l * l, 256) < 4096 ? ceil_div((long)l * l, 256) : 4096;
+  hipLaunchKernelGGL(sym_copy_kernel, dim3(gcopy), dim3(256), 0, st, Mm, l, ld, W);
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(gcopy), dim3(256), 0, st, L, (long)l * l);
+  DFTB_CHECK_LAUNCH();
+  int rc;
+  for (int p = 0; p < l; p += CHOL_NB) {
+    const int b = l - p < CHOL_NB ? l - p : CHOL_NB;
+    hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, st, (const double*)W, L, l, p, b, Linv, info);
+    DFTB_CHECK_LAUNCH();
+    const int r = l - p - b;
+    if (r <= 0) break;
+    // panel: L21 = W21 L11^{-T}
+    GemmArgs g = gemm_args_default();
+    g.A = W + (long)(p + b) * l + p; g.lda = l;
+    g.B = Linv; g.ldb = b; g.b_trans = 1;
+    g.C = L + (long)(p + b) * l + p; g.ldc = l;
+    g.M = r; g.N = b; g.K = b; g.tile = 64;
+    if ((rc = gemm_launch(g, st))) return rc;
+    // trailing: W22 -= L21 L21^T
+    GemmArgs u = gemm_args_default();
+    u.A = L + (long)(p + b) * l + p; u.lda = l;
+    u.B = L + (long)(p + b) * l + p; u.ldb = l; u.b_trans = 1;
+    u.C = W + (long)(p + b) * l + (p + b); u.ldc = l;
+    u.M = r; u.N = r; u.K = b; u.alpha = -1.0; u.beta = 1.0; u.tile = 64;
+    if ((rc = gemm_launch(u, st))) return rc;
+  }
+  return DFTB_OK;
+}
+
+extern "C" int dftb_subspace_nonhermitian_rl(const double* Mp, const double* Mm, int l, int ld, int nev, double* w,
+                                             double* Rb, double* Lb, int* info, double* diag, void* workspace,
+                                             size_t ws_bytes, void* stream) {
+  if (l <= 0 || nev <= 0 || nev > l || ld < l || l > EIG_MAXL) return DFTB_ERR_ARG;
+  if (!Mp || !Mm || !w || !Rb || !Lb || !info || !diag || !workspace) return DFTB_ERR_ARG;
+  SubLayout S = sub_layout(l, nev);
+  if (ws_bytes < S.total) return DFTB_ERR_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  double* W = (double*)(ws + S.Sq);      // Cholesky scratch, then T'
+  double* L = (double*)(ws + S.Sinv);
+  double* X = (double*)(ws + S.X);       // the diagonal blocks' inverses, then Mp L
+  double* H = (double*)(ws + S.H);
+  double* w2 = (double*)(ws + S.w2);
+  double* sR = (double*)(ws + S.sR);
+  double* sL = (double*)(ws + S.sL);
+  double* G = (double*)(ws + S.G);
+  int rc;
+  hipLaunchKernelGGL(zero_info_kernel, dim3(1), dim3(1), 0, st, info);
+  DFTB_CHECK_LAUNCH();
+  if ((rc = cholesky_blocked(Mm, l, ld, W, L, X, info, st))) return rc;
+  // H' = L^T Mp L, Hermiticity check
+  if ((rc = gemm_launch(small_gemm(Mp, ld, false, L, l, X, l, l, l, l), st))) return rc;
+  if ((rc = gemm_launch(small_gemm(L, l, true, X, l, H, l, l, l, l), st))) return rc;
+  {
+    const int g = red_blocks((long)l * l);
+    double* part = (double*)(ws + S.eig);
+    hipLaunchKernelGGL(herm_part_kernel, dim3(g), dim3(256), 0, st, (const double*)H, l, part);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(64), 0, st, (const double*)part, g, diag);
+    DFTB_CHECK_LAUNCH();
+  }
+  double* Tp = W;
+  if ((rc = dftb_syevx(H, l, l, nev, w2, Tp, ws + S.eig, eig_layout(l, nev).total, stream))) return rc;
+  hipLaunchKernelGGL(roots_kernel, dim3((nev + 63) / 64), dim3(64), 0, st, w2, nev, w, sR, sL, info);
+  DFTB_CHECK_LAUNCH();
+  // Rb = L T' / sqrt(w);  Lb = Mp Rb / w
+  if ((rc = gemm_launch(small_gemm(L, l, false, Tp, nev, Rb, nev, l, nev, l), st))) return rc;
+  hipLaunchKernelGGL(scale_cols_kernel, dim3((l * nev + 255) / 256), dim3(256), 0, st, Rb, l, nev, sR);
+  DFTB_CHECK_LAUNCH();
+  if ((rc = gemm_launch(small_gemm(Mp, ld, false, Rb, nev, Lb, nev, l, nev, l), st))) return rc;
+  hipLaunchKernelGGL(scale_cols_kernel, dim3((l * nev + 255) / 256), dim3(256), 0, st, Lb, l, nev, sL);
+  DFTB_CHECK_LAUNCH();
+  if ((rc = gemm_launch(small_gemm(Lb, nev, true, Rb, nev, G, nev, nev, nev, l), st))) return rc;
+  hipLaunchKernelGGL(biortho_kernel, dim3(1), dim3(256), 0, st, G, nev, diag);
+  DFTB_CHECK_LAUNCH();
+  return DFTB_OK;
+}
+
+__global__ void sqrt_vec_kernel(const double* w, int n, double* s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) s[i] = sqrt(fmax(w[i], 0.0));
+}
+
+extern "C" int dftb_subspace_tb(const double* Mm, int l, int ld, int nev, const double* w, const double* Rb,
+                                double* Tb, void* workspace, size_t ws_bytes, void* stream) {
+  if (l <= 0 || nev <= 0 || nev > l || ld < l || l > EIG_MAXL || !Mm || !w || !Rb || !Tb || !workspace)
+    return DFTB_ERR_ARG;
+  SubLayout S = sub_layout(l, nev);
+  if (ws_bytes < S.total) return DFTB_ERR_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  double* Sq = (double*)(ws + S.Sq);
+  double* Sinv = (double*)(ws + S.Sinv);
+  double* sw = (double*)(ws + S.sR);
+  int nsit = 0;
+  double nserr = 0.0;
+  int rc = sqrtm_ns(Mm, l, ld, Sq, Sinv, ws + S.eig, st, &nsit, &nserr);
+  if (rc) return rc;
+  // Tb = Mm^{-1/2} Rb diag(sqrt(w))
+  if ((rc = gemm_launch(small_gemm(Sinv, l, false, Rb, nev, Tb, nev, l, nev, l), st))) return rc;
+  hipLaunchKernelGGL(sqrt_vec_kernel, dim3((nev + 63) / 64), dim3(64), 0, st, w, nev, sw);
+  hipLaunchKernelGGL(scale_cols_kernel, dim3((l * nev + 255) / 256), dim3(256), 0, st, Tb, l, nev, (const double*)sw);
   DFTB_CHECK_LAUNCH();
   return DFTB_OK;
 }

@@ -711,7 +711,8 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
                         raise ExcitedStatesError("subspace matrix not symmetric (relative asymmetry %.3e)" % a)
             nb = _lib.load().dftb_subspace_workspace(l, k)
             wsb, wsn = ws.get(nb)
-            call("dftb_subspace_nonhermitian", _p(Mp), _p(Mm), l, cap, k, _p(wdev), _p(Tb), _p(Rb), _p(Lb),
+            # roots, Rb and Lb through a Cholesky factor of Mm; Tb (for C) once at the end
+            call("dftb_subspace_nonhermitian_rl", _p(Mp), _p(Mm), l, cap, k, _p(wdev), _p(Rb), _p(Lb),
                  _p(info), _p(diag), _p(wsb), wsn, st)
             inf = info.cpu().numpy()
             dg = diag.cpu().numpy()
@@ -810,7 +811,10 @@ def nonhermitian_davidson_gpu(op, nstates, ifact=1, conv=1.0e-14, maxiter=10, Xm
                                         refine=False)
         stats.t_total = time.time() - t_start
         return out
-    # C = bs Tb (Solver.py:906-910)
+    # Tb of the last subspace step (the eigenvectors of Mmsq Mp Mmsq that belong to its
+    # Rb), then C = bs Tb (Solver.py:906-910)
+    wsb, wsn = ws.get(_lib.load().dftb_subspace_workspace(l, k))
+    call("dftb_subspace_tb", _p(Mm), l, cap, k, _p(wdev), _p(Rb), _p(Tb), _p(wsb), wsn, st)
     C = torch.empty((k, N), dtype=torch.float64, device=dev)
     gemm(Tb, bs, C, k, N, l, k, N, N, a_trans=1, tile=64)
     stats.t_total = time.time() - t_start

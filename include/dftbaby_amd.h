@@ -235,6 +235,20 @@ size_t dftb_subspace_workspace(int l, int nev);
 int dftb_subspace_nonhermitian(const double* Mp, const double* Mm, int l, int ld, int nev,
                                double* w, double* Tb, double* Rb, double* Lb, int* info, double* diag,
                                void* workspace, size_t ws_bytes, void* stream);
+/* The same step by a Cholesky factor, for the iterations (Solver.py:759-798,
+ * with the roots and Rb, Lb of Mh = Mmsq Mp Mmsq obtained without Mmsq): with
+ * Mm = L L^T, H' = L^T Mp L has the roots w^2 of Mh; for H' T' = T' diag(w^2),
+ * Rb = L T' / sqrt(w) and Lb = Mp Rb / w equal those of
+ * dftb_subspace_nonhermitian.  Outputs w, Rb, Lb; info as there (info[1]: the
+ * failing pivot); diag[0] = sum |H' - H'^T|, diag[1] = sum |Lb^T Rb - 1|.
+ * Workspace: dftb_subspace_workspace(l, nev). */
+int dftb_subspace_nonhermitian_rl(const double* Mp, const double* Mm, int l, int ld, int nev,
+                                  double* w, double* Rb, double* Lb, int* info, double* diag,
+                                  void* workspace, size_t ws_bytes, void* stream);
+/* Tb = Mm^{-1/2} Rb diag(sqrt(w)): the eigenvectors of Mh belonging to the Rb of
+ * the Cholesky step (C = bs Tb, Solver.py:906-910).  Workspace as above. */
+int dftb_subspace_tb(const double* Mm, int l, int ld, int nev, const double* w, const double* Rb,
+                     double* Tb, void* workspace, size_t ws_bytes, void* stream);
 /* M[:l,:l] <- (M + M^T)/2; asym[0] = max|M - M^T| / max|M| (asym holds 3 doubles:
  * asym[1] and asym[2] receive the two maxima).  Used for the
  * subspace matrices in the mixed-precision modes, where the products carry

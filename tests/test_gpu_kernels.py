@@ -581,7 +581,7 @@ def test_hermitian_matvec(rng):
 # ---------------------------------------------------------------------------
 # (c) subspace kernels
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("l,nev", [(1, 1), (2, 2), (9, 4), (60, 20), (180, 12), (257, 30)])
+@pytest.mark.parametrize("l,nev", [(1, 1), (2, 2), (9, 4), (60, 20), (100, 20), (180, 12), (257, 30)])
 def test_syevx(l, nev, rng):
     from dftbaby_amd import _lib
     from dftbaby_amd.engine import _p, _stream, Workspace
@@ -632,6 +632,92 @@ def test_syevx_clustered_wanted_roots(l, nev, rng):
     np.testing.assert_allclose(e, ref, atol=1e-12 * np.abs(ev).max())
     np.testing.assert_allclose(Vh.T @ Vh, np.eye(nev), atol=1e-11)
     np.testing.assert_allclose(A @ Vh, Vh * e, atol=1e-10 * np.abs(ev).max())
+
+
+@pytest.mark.parametrize("layout", ["two_large", "isolated_then_large", "all_roots"])
+def test_syevx_several_clusters(layout, rng):
+    """Wanted roots in more than one re-orthogonalisation cluster (two clusters of
+    10 members, isolated roots before a cluster, every root of a spread spectrum):
+    every eigenvector is a unit residual-free vector, orthogonal to the others."""
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _p, _stream, Workspace
+    l = 200
+    if layout == "two_large":
+        ev, nev = np.concatenate([0.01 + 1e-3 * np.arange(10), 0.6 + 1e-3 * np.arange(10),
+                                  np.linspace(1.5, 40.0, l - 20)]), 24
+    elif layout == "isolated_then_large":
+        ev, nev = np.concatenate([[0.001, 0.2, 0.4], 0.6 + 1e-3 * np.arange(12), np.linspace(1.5, 40.0, l - 15)]), 24
+    else:
+        ev, nev = np.sort(np.concatenate([rng.uniform(0.0, 0.5, 60), rng.uniform(2.0, 3.0, l - 60)])), l
+    Q, _ = np.linalg.qr(rng.standard_normal((l, l)))
+    A = (Q * ev) @ Q.T
+    A = 0.5 * (A + A.T)
+    e_d = torch.empty(nev, dtype=torch.float64, device=_dev())
+    V = torch.empty((l, nev), dtype=torch.float64, device=_dev())
+    w, wn = Workspace(_dev()).get(_lib.load().dftb_syevx_workspace(l, nev))
+    _lib.call("dftb_syevx", _p(T(A)), l, l, nev, _p(e_d), _p(V), _p(w), wn, _stream())
+    e, Vh = e_d.cpu().numpy(), V.cpu().numpy()
+    np.testing.assert_allclose(e, np.sort(ev)[:nev], atol=1e-12 * np.abs(ev).max())
+    np.testing.assert_allclose(Vh.T @ Vh, np.eye(nev), atol=1e-11)
+    np.testing.assert_allclose(A @ Vh, Vh * e, atol=1e-10 * np.abs(ev).max())
+
+
+@pytest.mark.parametrize("l,nev", [(40, 8), (150, 20), (333, 20)])
+def test_subspace_cholesky_route_equals_sqrtm_route(l, nev, rng):
+    """dftb_subspace_nonhermitian_rl (Cholesky factor of Mm) and dftb_subspace_tb
+    give the roots, Rb, Lb and Tb of dftb_subspace_nonhermitian (square root of
+    Mm, Solver.py:759-798), columns up to a common sign; an indefinite Mm is
+    reported as not positive definite."""
+    from dftbaby_amd import _lib
+    from dftbaby_amd.engine import _p, _stream, Workspace
+
+    def spd(lo, hi):
+        Q, _ = np.linalg.qr(rng.standard_normal((l, l)))
+        A = (Q * np.linspace(lo, hi, l)) @ Q.T
+        return 0.5 * (A + A.T)
+
+    Mp, Mm = spd(0.05, 3.0), spd(0.02, 2.0)
+    ld = l + 5
+    Mpd = torch.zeros((l, ld), dtype=torch.float64, device=_dev())
+    Mmd = torch.zeros((l, ld), dtype=torch.float64, device=_dev())
+    Mpd[:, :l] = T(Mp)
+    Mmd[:, :l] = T(Mm)
+    w, wn = Workspace(_dev()).get(_lib.load().dftb_subspace_workspace(l, nev))
+    out = {}
+    for route in ("sqrtm", "chol"):
+        wd = torch.zeros(nev, dtype=torch.float64, device=_dev())
+        Tb, Rb, Lb = (torch.zeros((l, nev), dtype=torch.float64, device=_dev()) for _ in range(3))
+        info = torch.zeros(2, dtype=torch.int32, device=_dev())
+        diag = torch.zeros(2, dtype=torch.float64, device=_dev())
+        if route == "sqrtm":
+            _lib.call("dftb_subspace_nonhermitian", _p(Mpd), _p(Mmd), l, ld, nev, _p(wd), _p(Tb), _p(Rb), _p(Lb),
+                      _p(info), _p(diag), _p(w), wn, _stream())
+        else:
+            _lib.call("dftb_subspace_nonhermitian_rl", _p(Mpd), _p(Mmd), l, ld, nev, _p(wd), _p(Rb), _p(Lb),
+                      _p(info), _p(diag), _p(w), wn, _stream())
+            _lib.call("dftb_subspace_tb", _p(Mmd), l, ld, nev, _p(wd), _p(Rb), _p(Tb), _p(w), wn, _stream())
+        assert list(info.cpu().numpy()) == [0, 0]
+        assert diag.cpu().numpy()[1] < 1e-10
+        out[route] = [x.cpu().numpy() for x in (wd, Tb, Rb, Lb)]
+    ws_, Tq, Rq, Lq = out["sqrtm"]
+    wc, Tc, Rc, Lc = out["chol"]
+    ref = np.sqrt(np.sort(np.linalg.eigvals(Mp @ Mm).real)[:nev])
+    np.testing.assert_allclose(ws_, ref, rtol=1e-12)
+    np.testing.assert_allclose(wc, ref, rtol=1e-12)
+    sgn = np.sign(np.sum(Tq * Tc, axis=0))
+    for a, b in ((Tq, Tc), (Rq, Rc), (Lq, Lc)):
+        np.testing.assert_allclose(a, b * sgn, atol=1e-10 * np.abs(a).max())
+    # Tb orthonormal eigenvectors of Mh = Mm^1/2 Mp Mm^1/2
+    np.testing.assert_allclose(Tc.T @ Tc, np.eye(nev), atol=1e-11)
+    # indefinite Mm
+    Mmd[:, :l] = T(Mm - 0.5 * np.eye(l))
+    info = torch.zeros(2, dtype=torch.int32, device=_dev())
+    diag = torch.zeros(2, dtype=torch.float64, device=_dev())
+    Rb, Lb = (torch.zeros((l, nev), dtype=torch.float64, device=_dev()) for _ in range(2))
+    wd = torch.zeros(nev, dtype=torch.float64, device=_dev())
+    _lib.call("dftb_subspace_nonhermitian_rl", _p(Mpd), _p(Mmd), l, ld, nev, _p(wd), _p(Rb), _p(Lb),
+              _p(info), _p(diag), _p(w), wn, _stream())
+    assert info.cpu().numpy()[0] == _lib.DFTB_ERR_NOT_SPD
 
 
 def test_sqrtm(rng):
