@@ -670,60 +670,70 @@ __global__ void fill_zero_kernel(double* X, long n) {
 }
 
 // Factor the b x b diagonal block at (p, p) of W (lower triangle, trailing updates
-// applied): L[p.., p..] = its Cholesky factor (zeros above the diagonal), Linv = the
-// factor's inverse (b x b).  A non-positive pivot sets info (DFTB_ERR_NOT_SPD, pivot
-// index) and continues with a unit pivot, so that later launches see finite values.
-__global__ __launch_bounds__(256) void chol_diag_kernel(const double* __restrict__ W, double* __restrict__ L, int l,
-                                                        int p, int b, double* __restrict__ Linv, int* __restrict__ info) {
-  __shared__ double sA[CHOL_NB][CHOL_NB + 1];
-  __shared__ double sI[CHOL_NB][CHOL_NB + 1];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < b * b; i += blockDim.x) {
-    const int r = i / b, c = i % b;
-    sA[r][c] = (c <= r) ? W[(long)(p + r) * l + (p + c)] : 0.0;
-  }
-  __syncthreads();
-  for (int k = 0; k < b; ++k) {
-    if (tid == 0) {
-      double dk = sA[k][k];
-      if (!(dk > 0.0)) {
-        if (info[0] == 0) {
-          info[0] = DFTB_ERR_NOT_SPD;
-          info[1] = p + k;
-        }
-        dk = 1.0;
+// applied) into L[p.., p..] (zeros above the diagonal).  One wave: lane i holds row
+// i in registers and column k travels by lane shuffles.  A non-positive pivot sets
+// info (DFTB_ERR_NOT_SPD, pivot index) and continues with a unit pivot, so that
+// later launches see finite values.
+__global__ __launch_bounds__(64) void chol_diag_kernel(const double* __restrict__ W, double* __restrict__ L, int l,
+                                                       int p, int b, int* __restrict__ info) {
+  const int i = threadIdx.x;
+  double row[CHOL_NB];
+#pragma unroll
+  for (int j = 0; j < CHOL_NB; ++j) row[j] = (i < b && j <= i) ? W[(long)(p + i) * l + (p + j)] : 0.0;
+  int bad = -1;
+#pragma unroll
+  for (int k = 0; k < CHOL_NB; ++k) {
+    if (k < b) {
+      double d = __shfl(row[k], k, 64);
+      if (!(d > 0.0)) {
+        if (bad < 0) bad = k;
+        d = 1.0;
       }
-      sA[k][k] = sqrt(dk);
-    }
-    __syncthreads();
-    const double pk = sA[k][k];
-    for (int i = k + 1 + tid; i < b; i += blockDim.x) sA[i][k] /= pk;
-    __syncthreads();
-    const int n = b - k - 1;
-    for (int t = tid; t < n * n; t += blockDim.x) {
-      const int i = k + 1 + t / n, j = k + 1 + t % n;
-      if (j <= i) sA[i][j] -= sA[i][k] * sA[j][k];
-    }
-    __syncthreads();
-  }
-  // inverse by forward substitution, one column per thread
-  for (int c = tid; c < b; c += blockDim.x) {
-    for (int i = 0; i < b; ++i) {
-      if (i < c) {
-        sI[i][c] = 0.0;
-        continue;
+      const double piv = sqrt(d);
+      const double lik = (i == k) ? piv : (i > k ? row[k] / piv : 0.0);
+      row[k] = lik;
+#pragma unroll
+      for (int j = k + 1; j < CHOL_NB; ++j) {
+        const double ljk = __shfl(lik, j, 64);
+        if (i >= j) row[j] -= lik * ljk;
       }
-      double s = (i == c) ? 1.0 : 0.0;
-      for (int k = c; k < i; ++k) s -= sA[i][k] * sI[k][c];
-      sI[i][c] = s / sA[i][i];
     }
   }
-  __syncthreads();
-  for (int i = tid; i < b * b; i += blockDim.x) {
-    const int r = i / b, c = i % b;
-    L[(long)(p + r) * l + (p + c)] = (c <= r) ? sA[r][c] : 0.0;
-    Linv[i] = sI[r][c];
+  if (i == 0 && bad >= 0 && info[0] == 0) {
+    info[0] = DFTB_ERR_NOT_SPD;
+    info[1] = p + bad;
   }
+  if (i < b)
+#pragma unroll
+    for (int j = 0; j < CHOL_NB; ++j)
+      if (j < b) L[(long)(p + i) * l + (p + j)] = (j <= i) ? row[j] : 0.0;
+}
+
+// Linv (b x b) = inverse of the factor block L[p.., p..]: one wave, lane c forms
+// column c by column-oriented forward substitution (rows of the factor read from
+// LDS at uniform addresses); entries above the diagonal come out zero.
+__global__ __launch_bounds__(64) void chol_inv_kernel(const double* __restrict__ L, int l, int p, int b,
+                                                      double* __restrict__ Linv) {
+  __shared__ double sL[CHOL_NB][CHOL_NB + 1];
+  const int i = threadIdx.x;
+  for (int j = 0; j < CHOL_NB; ++j) sL[i][j] = (i < b && j < b) ? L[(long)(p + i) * l + (p + j)] : 0.0;
+  __syncthreads();
+  double x[CHOL_NB];
+#pragma unroll
+  for (int r = 0; r < CHOL_NB; ++r) x[r] = (r == i) ? 1.0 : 0.0;
+#pragma unroll
+  for (int k = 0; k < CHOL_NB; ++k) {
+    if (k < b) {
+      const double xk = x[k] / sL[k][k];
+      x[k] = xk;
+#pragma unroll
+      for (int r = k + 1; r < CHOL_NB; ++r) x[r] -= sL[r][k] * xk;
+    }
+  }
+  if (i < b)
+#pragma unroll
+    for (int r = 0; r < CHOL_NB; ++r)
+      if (r < b) Linv[r * b + i] = x[r];
 }
 
 // L (l x l, zero above the diagonal) with L L^T = sym(Mm); W: l x l scratch.
@@ -736,10 +746,12 @@ static int cholesky_blocked(const double* Mm, int l, int ld, double* W, double* 
   int rc;
   for (int p = 0; p < l; p += CHOL_NB) {
     const int b = l - p < CHOL_NB ? l - p : CHOL_NB;
-    hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, st, (const double*)W, L, l, p, b, Linv, info);
+    hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(64), 0, st, (const double*)W, L, l, p, b, info);
     DFTB_CHECK_LAUNCH();
     const int r = l - p - b;
     if (r <= 0) break;
+    hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(64), 0, st, (const double*)L, l, p, b, Linv);
+    DFTB_CHECK_LAUNCH();
     // panel: L21 = W21 L11^{-T}
     GemmArgs g = gemm_args_default();
     g.A = W + (long)(p + b) * l + p; g.lda = l;
