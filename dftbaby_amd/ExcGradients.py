@@ -6,7 +6,8 @@ the long-range corrected case: ``Gradients(tddftb).gradient(I)`` returns
 energy of state I = their sum).  The terms are the reference's (``gradient_lc``):
 the ground-state gradient (``:477-496``), the relaxed excitation-energy gradient
 from U, V, T, the H^+ / H^- contractions, Q, the Z-vector equation (solved by
-``engine.zvector_gpu`` on the (A+B) operator), W, and the contractions with the
+``engine.zvector_gpu`` on the (A+B) operator, in the precision of the excited-state
+solve with FP64 refinement to the reference's residual test), W, and the contractions with the
 derivatives of S, H0, gamma and gamma_lr (``:590-711``).
 
 What differs is how they are formed (identical results to rounding):
@@ -62,6 +63,10 @@ def _mm(A, B, ta=False, tb=False, alpha=1.0, out=None, beta=0.0):
 
 class Gradients:
     """Gradients of the excited states of an LR_TDDFTB instance (ExcGradients.py:213)."""
+
+    # products of the Z-vector solve: None = those of the excited-state solve
+    # (precision code 0 FP64, 1 FP32, 2 BF16x3)
+    zvector_precision = None
 
     def __init__(self, tddftb):
         self.tddftb = tddftb
@@ -317,7 +322,12 @@ class Gradients:
         # Z-vector equation (ExcGradients.py:620-640)
         op = self._zvector_operator()
         guess = self.Zia[None] if (self.I == I and self.Zia is not None) else None
-        Zia_h = zvector_gpu(op, Ria[None].cpu().numpy(), X0=guess)[0]
+        # the Z-vector in the precision of the excited-state solve (FP32 / BF16x3 products
+        # with FP64 refinement to the same residual test), FP64 otherwise
+        zprec = self.zvector_precision
+        if zprec is None:
+            zprec = getattr(self.tddftb, "solve_precision", 0) if op is getattr(self.tddftb, "op", None) else 0
+        Zia_h = zvector_gpu(op, Ria[None].cpu().numpy(), X0=guess, precision=zprec)[0]
         self.Zia, self.I = Zia_h, I
         Zia = torch.as_tensor(Zia_h, device=self.dev)
         # W (ExcGradients.py:672-682)

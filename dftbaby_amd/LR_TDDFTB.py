@@ -243,6 +243,7 @@ class LR_TDDFTB:
         op = self.op
         lc = op.lc
         prec = precision_code(precision)
+        self.solve_precision = prec           # the Z-vector solve of the gradients follows it
         # stability of the ground state (LR_TDDFTB.py:423-427)
         if stability == 1:
             from .StabilityAnalysis import check_stability

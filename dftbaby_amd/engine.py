@@ -1047,12 +1047,52 @@ def zvector_gpu(op, B, X0=None, maxiter=1000, conv=1.0e-14, precision=0, verbose
     The basis is kept orthonormal (same span, hence the same X, as the reference's
     Gram-Schmidt of the whole set).  B and X0 are (k, o, v); returns X (k, o, v).
     Raises Exception (the reference's) when maxiter iterations do not converge.
+
+    precision > 0 (FP32 / BF16x3 products): iterative refinement with the same
+    convergence test.  Each outer step forms the residual R = B - (A+B) X with FP64
+    products and solves (A+B) D = R with the Krylov iteration above on the
+    low-precision products to 1e-5 of |R| (their noise floor is ~1e-7 relative),
+    X += D; it stops when every FP64 residual norm is below conv.
     """
+    Bd = op.to_padded(B)
+    X0d = op.to_padded(X0) if X0 is not None else None
+    if precision == 0:
+        X, ok, it = _zvector_krylov(op, Bd, X0d, maxiter, conv, 0, verbose)
+        if not ok:
+            raise Exception("Z-vector equations: Krylov solver for linear system did not converge after %s "
+                            "iterations (threshold: %e) !!!" % (it, conv))
+        return op.from_padded(X).cpu().numpy().copy()
+    ws = op.ws
+    k, N = Bd.shape
+    X = X0d.clone() if X0d is not None else torch.zeros((k, N), dtype=torch.float64, device=op.device)
+    norms = torch.empty(k, dtype=torch.float64, device=op.device)
+    used = 0
+    for outer in range(40):
+        R = Bd - op.matvec(X, 0)[0] if (X0d is not None or outer > 0) else Bd.clone()
+        row_norms(R, k, N, norms, ws)
+        nr = norms.cpu().numpy()
+        if verbose > 0:
+            print("Z-vector refinement %d: FP64 residual norms %s" % (outer, nr))
+        if (nr < conv).all():
+            return op.from_padded(X).cpu().numpy().copy()
+        if used >= maxiter:
+            break
+        D, _, it = _zvector_krylov(op, R, None, min(300, maxiter - used), max(conv, 1.0e-5 * float(nr.max())),
+                                   precision, verbose)
+        used += it
+        X += D
+    raise Exception("Z-vector equations: Krylov solver for linear system did not converge after %s "
+                    "iterations (threshold: %e) !!!" % (used, conv))
+
+
+def _zvector_krylov(op, Bd, X0d, maxiter, conv, precision, verbose):
+    """The Krylov iteration of zvector_gpu on padded device arrays: returns
+    (X [k][N], converged, iterations); a basis that stops growing (no residual
+    above 0.01 conv, or no independent new vector) ends it unconverged."""
     dev = op.device
     st = _stream()
     ws = op.ws
     N, o, v = op.N, op.nocc, op.nvirt
-    Bd = op.to_padded(B)
     k = Bd.shape[0]
     kmax = o * v
     ainv = torch.where(op.omega_real.reshape(-1) != 0, 1.0 / op.omega, torch.zeros_like(op.omega))
@@ -1075,16 +1115,17 @@ def zvector_gpu(op, B, X0=None, maxiter=1000, conv=1.0e-14, precision=0, verbose
                 P = t
         cap = new
 
-    if X0 is None:
+    if X0d is None:
         call("dftb_scale2", _p(Bd), None, _p(ainv), k, N, _p(bs[:k]), st)
     else:
-        bs[:k].copy_(op.to_padded(X0))
+        bs[:k].copy_(X0d)
     l = _orthonormalize_block(bs, 0, k, N, ws, st)
     op.matvec(bs[:l], precision, out_p=P[:l])
     norms = torch.empty(k, dtype=torch.float64, device=dev)
     X = torch.empty((k, N), dtype=torch.float64, device=dev)
     W = torch.empty((k, N), dtype=torch.float64, device=dev)
     t0 = time.time()
+    it = 0
     for it in range(maxiter):
         Ab = torch.empty((l, l), dtype=torch.float64, device=dev)
         gram(bs, l, P, l, N, Ab, ws)
@@ -1117,7 +1158,7 @@ def zvector_gpu(op, B, X0=None, maxiter=1000, conv=1.0e-14, precision=0, verbose
         if (nr < conv).all():
             if verbose > 0:
                 print("Z-vector equations CONVERGED (residual norms < %s) after %d iterations" % (conv, it + 1))
-            return op.from_padded(X).cpu().numpy().copy()
+            return X, True, it + 1
         dkmax = min(kmax - l, k)
         sel = [n for n in range(dkmax) if nr[n] > 0.01 * conv]
         if not sel:
@@ -1130,8 +1171,7 @@ def zvector_gpu(op, B, X0=None, maxiter=1000, conv=1.0e-14, precision=0, verbose
             break
         op.matvec(bs[l:l_new], precision, out_p=P[l:l_new])
         l = l_new
-    raise Exception("Z-vector equations: Krylov solver for linear system did not converge after %s iterations "
-                    "(threshold: %e) !!!" % (it + 1, conv))
+    return X, False, it + 1
 
 
 def stability_gpu(q_oo, q_vv, q_ov, omega, gamma, gamma_lr, nev=6, chunk=512):

@@ -200,6 +200,29 @@ def test_stability_analysis(lc, capsys):
     assert "lowest eigenvalue of restricted/unrestricted stability matrix R" in capsys.readouterr().out
 
 
+@pytest.mark.parametrize("precision", [1, 2])
+def test_zvector_mixed_precision_refinement(precision, rng):
+    """zvector_gpu with FP32 / BF16x3 products and FP64 refinement: the solution
+    of the dense (A+B) X = rhs to the FP64 residual test, on the AO operator."""
+    from dftbaby_amd.engine import CasidaOperatorGPU, zvector_gpu
+    gs = ground_state("C12H14", 1, 0.05, 5)
+    f, orbe, C, S = gs.getOccupation(), gs.getKSEnergies(), gs.getKSCoefficients(), gs.getOverlapMatrix()
+    _, _, ao, av = O.active_orbitals(f, orbe)
+    q_oo, q_ov, q_vv = O.trans_charges(C[:, ao], C[:, av], S, gs.orbsPerAtom)
+    omega, df = O.orbital_differences(orbe, f, ao, av)
+    g, glr = gs.getGamma(), gs.getGamma_lr()
+    o, v = omega.shape
+    A, B = O.buildAB(g, glr, q_oo, q_vv, q_ov, omega, 0 * omega, df, o, v)
+    rhs = rng.standard_normal((2, o, v))
+    Xd = np.linalg.solve(A + B, rhs.reshape(2, o * v).T).T.reshape(2, o, v)
+    op = CasidaOperatorGPU.from_orbitals(C[:, ao], C[:, av], S, gs.orbsPerAtom, omega, g, glr, 1, exchange="ao")
+    X = zvector_gpu(op, rhs, conv=1e-11, precision=precision)
+    assert np.abs(X - Xd).max() < 1e-9 * np.abs(Xd).max()
+    P, _ = op.matvec(op.to_padded(X), 0)
+    res = np.linalg.norm((op.from_padded(P).cpu().numpy() - rhs).reshape(2, -1), axis=1)
+    assert res.max() < 1e-11
+
+
 @pytest.mark.parametrize("lc", [0, 1])
 def test_dense_casida_triplets(lc):
     """nstates=None with multiplicity="T": the dense Casida of the triplet matrices

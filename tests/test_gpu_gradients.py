@@ -45,6 +45,24 @@ def test_gradients_match_oracle(name, seed, lc):
         progress("%s state %d: max |grad| %.3e" % (name, I, np.abs(gExc if I else gE0).max()))
 
 
+def test_mixed_precision_zvector_gradient():
+    """After a BF16x3 excited-state solve (refined to diag_conv 1e-10) the Z-vector
+    equations are solved with BF16x3 products and FP64 refinement to the same
+    residual test: Z and the excitation gradient equal the oracle's on the same
+    X-Y, X+Y."""
+    from dftbaby_amd.ExcGradients import Gradients
+    from dftbaby_amd.LR_TDDFTB import LR_TDDFTB
+    gs = ground_state("C12H14", 1, 0.05, 5)
+    td = LR_TDDFTB(ground_state=gs)
+    td.getEnergies(nstates=3, diag_conv=1e-10, diag_maxiter=100, precision="bf16x3")
+    assert td.solve_precision == 2
+    grads = Gradients(td)
+    gV, gE0, gExc = grads.gradient(I=1)
+    rE0, rExc, rZ = G.gradient_lc(gs, td.XmY[0], td.XpY[0], td.Omega[0])
+    np.testing.assert_allclose(grads.Zia, rZ, rtol=0, atol=1e-10)
+    np.testing.assert_allclose(gExc, rExc, rtol=0, atol=1e-9 * max(1.0, np.abs(rExc).max()))
+
+
 def test_gradient_with_restricted_active_space():
     """nr_active_occ / nr_active_virt: the gradient uses the amplitudes padded
     with zeros over the full space (the reference's own treatment)."""
