@@ -342,7 +342,7 @@ static int check_op(const CasidaOperator* op) {
   if (op->nat <= 0 || op->nocc <= 0 || op->nvirt <= 0) return DFTB_ERR_ARG;
   if (op->opad < op->nocc || op->vpad < op->nvirt || (op->opad & 7) || (op->vpad & 7)) return DFTB_ERR_ARG;
   if (op->precision < 0 || op->precision > 2) return DFTB_ERR_ARG;
-  if (!op->omega || !op->gamma || !op->q_c) return DFTB_ERR_ARG;
+  if (!op->omega || !op->gamma) return DFTB_ERR_ARG;
   if (op->lc && op->exchange == 1) {
     if (op->npad <= 0 || (op->npad & 7) || !op->gam_ao || !op->amap) return DFTB_ERR_ARG;
     if (!op->ao_v[0] || !op->ao_o[0] || !op->ao_cyvt[0] || !op->ao_w[0]) return DFTB_ERR_ARG;
@@ -395,7 +395,13 @@ extern "C" int dftb_casida_matvec(const CasidaOperator* op, const double* V, int
       }
     }
   }
-  if ((rc = coulomb(op, L, ws, Vc, nvec, 4.0, ApBV, st))) return rc;
+  // q_c = NULL: the Coulomb term is left to the caller ((A+B)V starts from zero)
+  if (op->q_c) {
+    if ((rc = coulomb(op, L, ws, Vc, nvec, 4.0, ApBV, st))) return rc;
+  } else {
+    hipLaunchKernelGGL(zero_kernel, sgrid(total), dim3(256), 0, st, ApBV, total);
+    DFTB_CHECK_LAUNCH();
+  }
 
   if (op->lc) {
     double* X1 = (double*)(ws + L.x1);

@@ -397,7 +397,10 @@ class CasidaOperatorGPU:
 
     # -- products ---------------------------------------------------------------
     def matvec(self, V, precision=0, out_p=None, out_m=None, atom_chunk=0, tiles=None, atom_limit=0):
-        """((A+B)V, (A-B)V) for V [m][opad*vpad] (FP64, zero padding)."""
+        """((A+B)V, (A-B)V) for V [m][opad*vpad] (FP64, zero padding).  With the AO
+        exchange, blocks of at most AO_COULOMB_MAX vectors (the Z-vector solve) take
+        the Coulomb term from the AO operands in FP64 (_coulomb_ao) instead of two
+        passes over the [nat][N] charges."""
         m = V.shape[0]
         if out_p is None:
             out_p = torch.empty((m, self.N), dtype=torch.float64, device=self.device)
@@ -406,10 +409,42 @@ class CasidaOperatorGPU:
         if m == 0:
             return out_p, out_m
         s = self._struct(precision, atom_chunk, tiles, atom_limit)
+        ao_coulomb = m <= self.AO_COULOMB_MAX and self.lc and self.exchange == "ao" and atom_limit == 0
+        if ao_coulomb:
+            s.q_c = None                      # the library leaves the Coulomb term out
         nb = _lib.load().dftb_casida_matvec_workspace(ctypes.byref(s), m)
         w, wn = self.ws.get(nb)
         call("dftb_casida_matvec", ctypes.byref(s), _p(V), m, _p(out_p), _p(out_m), _p(w), wn, _stream())
+        if ao_coulomb:
+            self._coulomb_ao(V, out_p)
         return out_p, out_m
+
+    AO_COULOMB_MAX = 2
+
+    def _coulomb_ao(self, V, out_p):
+        """out_p += 4 q^T gamma q V through the AO operands, with the transition charges
+        q^{ia}_A = 1/2 sum_{mu in A} (C_o,mu,i Y_v,mu,a + Y_o,mu,i C_v,mu,a), Y = S C:
+        u_A = 1/2 sum_{mu in A} rowdot([C_o; Y_o] V, [Y_v; C_v])_mu, g = 4 gamma u,
+        W = 1/2 [Y_o; C_o]^T [g C_v; g Y_v] -- two FP64 GEMMs of 2 npad x opad x vpad
+        per vector instead of streaming the charges twice."""
+        m = V.shape[0]
+        npad, nat, op_, vp = self.npad, self.nat, self.opad, self.vpad
+        dev = self.device
+        if getattr(self, "_ao_cv", None) is None:
+            self._ao_cv = self.ao["cyvt"].T.contiguous()            # [C_v; Y_v]
+            self._amap_l = self.ao["amap"].to(torch.int64)
+        X = torch.empty((m, 2 * npad, vp), dtype=torch.float64, device=dev)
+        gemm(self.ao["w"], V, X, 2 * npad, vp, op_, 2 * npad, vp, vp, a_trans=1, batch=m, sB=self.N,
+             sC=2 * npad * vp)                                     # [C_o; Y_o] V
+        r = (X * self.ao["v"][None]).sum(2)                        # rowdot with [Y_v; C_v]
+        r = 0.5 * (r[:, :npad] + r[:, npad:])
+        u = torch.zeros((m, nat + 1), dtype=torch.float64, device=dev).index_add_(1, self._amap_l, r)
+        g = torch.zeros((m, nat + 1), dtype=torch.float64, device=dev)
+        g[:, :nat] = 4.0 * (u[:, :nat] @ self.gamma)
+        gmu = g[:, self._amap_l]                                   # per orbital, padding 0
+        B = (torch.cat([gmu, gmu], 1)[:, :, None] * self._ao_cv[None]).contiguous()
+        gemm(self.ao["o"], B, out_p, op_, vp, 2 * npad, op_, vp, vp, a_trans=1, alpha=0.5, beta=1.0, batch=m,
+             sB=2 * npad * vp, sC=self.N)
 
     def hermitian_matvec(self, V, sqrt_omega, om, precision=0, out=None):
         m = V.shape[0]
@@ -1040,8 +1075,8 @@ def zvector_gpu(op, B, X0=None, maxiter=1000, conv=1.0e-14, precision=0, verbose
     the Z-vector equations of ExcGradients.py:627-635).
 
     Block Krylov (Galerkin) iteration: the basis starts from diag(omega)^{-1} B (or
-    X0), the projected system bs^T (A+B) bs Xb = bs^T B is solved through the GPU
-    eigensolver (it is symmetric positive definite), X = bs^T Xb, the residual
+    X0), the projected system bs^T (A+B) bs Xb = bs^T B is solved by a Cholesky
+    factorisation (it is symmetric positive definite), X = bs^T Xb, the residual
     (A+B) X - B comes from the stored products (no extra matvec), and each
     residual above 0.01 conv, preconditioned by diag(omega)^{-1}, expands the basis.
     The basis is kept orthonormal (same span, hence the same X, as the reference's
@@ -1133,18 +1168,12 @@ def _zvector_krylov(op, Bd, X0d, maxiter, conv, precision, verbose):
         gram(bs, l, Bd, k, N, Bb, ws)
         asym = torch.zeros(3, dtype=torch.float64, device=dev)
         call("dftb_symmetrize", _p(Ab), l, l, _p(asym), st)
-        # Xb = V diag(1/lambda) V^T Bb with Ab = V diag(lambda) V^T (GPU eigensolver)
-        lam = torch.empty(l, dtype=torch.float64, device=dev)
-        V = torch.empty((l, l), dtype=torch.float64, device=dev)
-        w_, wn_ = ws.get(_lib.load().dftb_syevx_workspace(l, l))
-        call("dftb_syevx", _p(Ab), l, l, l, _p(lam), _p(V), _p(w_), wn_, st)
-        if float(lam.min().cpu()) <= 0.0:
+        # Xb = Ab^{-1} Bb (la.solve, Solver.py:969) by a Cholesky factorisation of the
+        # symmetric positive definite projected (A+B) (ROCm solver library)
+        Lc, info = torch.linalg.cholesky_ex(Ab)
+        if int(info.cpu()) != 0:
             raise ExcitedStatesError("Z-vector equations: the projected (A+B) is not positive definite")
-        C1 = torch.empty((l, k), dtype=torch.float64, device=dev)
-        gemm(V, Bb, C1, l, k, l, l, k, k, a_trans=1)
-        call("dftb_scale2", _p(C1), _p(1.0 / lam), None, l, k, _p(C1), st)
-        Xb = torch.empty((l, k), dtype=torch.float64, device=dev)
-        gemm(V, C1, Xb, l, k, l, l, k, k)
+        Xb = torch.cholesky_solve(Bb, Lc).contiguous()
         gemm(Xb, bs, X, k, N, l, k, N, N, a_trans=1)
         gemm(Xb, P, W, k, N, l, k, N, N, a_trans=1)              # (A+B) X from the stored products
         W.sub_(Bd)

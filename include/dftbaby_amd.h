@@ -143,7 +143,9 @@ typedef struct CasidaOperator {
   int atom_limit;  /* > 0: exchange over the first atom_limit atoms only (autotuner timing probe) */
   const double* omega;  /* [opad*vpad], padding 1.0 */
   const double* gamma;  /* [nat*nat] */
-  const void* q_c;      /* [nat][opad][vpad]  f64 (precision 0) or f32 */
+  const void* q_c;      /* [nat][opad][vpad]  f64 (precision 0) or f32; NULL in
+                           dftb_casida_matvec: the Coulomb term is left out of
+                           (A+B)V (the caller adds it) */
   const void* q_oo[2];  /* [nocc][nat][opad] */
   const void* q_ov[2];  /* [nocc][nat][vpad] */
   const void* qf_ov[2]; /* [nocc][nat][vpad] gamma_lr folded */

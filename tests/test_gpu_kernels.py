@@ -511,6 +511,29 @@ def _operator(name, lc, sigma=0.05, seed=3, exchange="ao"):
     return op, (g, glr, q_oo, q_ov, q_vv, omega)
 
 
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (2, 3e-5)])
+def test_casida_matvec_small_block_ao_coulomb(precision, tol, rng):
+    """Blocks of one or two vectors take the Coulomb term through the AO operands
+    (the library is asked to leave it out): the same products as the oracle, and in
+    FP64 the same as the charge route of a larger block."""
+    op, (g, glr, q_oo, q_ov, q_vv, omega) = _operator("C12H14", 1, exchange="ao")
+    o, v = omega.shape
+    vs = rng.standard_normal((o, v, 5))
+    refP = O.ApBv(g, glr, q_oo, q_ov, q_vv, omega, vs, 1)
+    refM = O.AmBv(g, glr, q_oo, q_ov, q_vv, omega, vs, 1)
+    V = op.to_padded(np.moveaxis(vs, 2, 0))
+    P5, M5 = op.matvec(V, precision)
+    for sl in (slice(0, 1), slice(1, 3)):
+        P, M = op.matvec(V[sl].contiguous(), precision)
+        gotP = np.moveaxis(op.from_padded(P).cpu().numpy(), 0, 2)
+        gotM = np.moveaxis(op.from_padded(M).cpu().numpy(), 0, 2)
+        scale = np.abs(refP).max()
+        np.testing.assert_allclose(gotP, refP[:, :, sl], rtol=0, atol=tol * scale)
+        np.testing.assert_allclose(gotM, refM[:, :, sl], rtol=0, atol=tol * scale)
+        if precision == 0:
+            np.testing.assert_allclose(P.cpu().numpy(), P5[sl].cpu().numpy(), rtol=0, atol=1e-12 * scale)
+
+
 @pytest.mark.parametrize("exchange", ["ao", "charges"])
 @pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 3e-5), (2, 3e-5)])
 def test_casida_matvec(exchange, precision, tol, rng):
