@@ -302,6 +302,8 @@ int gemm_launch(const GemmArgs& p0, hipStream_t st, bool timed) {
     // per tile instead of 64 cuts the padded MFMA work by a quarter)
     if (small && p.M <= 48 && p.M > 32 && p.N >= 4096) launch_fp<float, 48, 128, 32, 1>(p, st);
     else if (small && p.M <= 48 && p.M > 32) launch_fp<float, 48, 64, 32, 1>(p, st);
+    else if (small && p.M <= 16) launch_fp<float, 16, 64, 32, 1>(p, st);
+    else if (small && p.M <= 32) launch_fp<float, 32, 64, 32, 1>(p, st);
     else if (small) launch_fp<float, 64, 64, 32>(p, st);
     else launch_fp<float, 128, 128, 32>(p, st);
   }
