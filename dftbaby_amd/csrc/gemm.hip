@@ -289,7 +289,8 @@ int gemm_launch(const GemmArgs& p0, hipStream_t st, bool timed) {
     // tall-skinny products of the Davidson basis (one side 20-40 vectors, the other
     // hundreds of basis vectors or the pair index): narrow tiles cut the padded
     // MFMA work, which bounds these calls at the FP64 MFMA rate
-    if (small && p.M <= 32 && p.N > 48) launch_fp<double, 32, 64, 16, 1>(p, st);
+    if (small && p.M <= 16 && p.N > 48) launch_fp<double, 16, 64, 16, 1>(p, st);
+    else if (small && p.M <= 32 && p.N > 48) launch_fp<double, 32, 64, 16, 1>(p, st);
     else if (small && p.M <= 48 && p.N > 48) launch_fp<double, 48, 64, 16, 1>(p, st);
     else if (small && p.N <= 32 && p.M > 48) launch_fp<double, 64, 32, 16, 4>(p, st);
     else if (small && p.N <= 48 && p.M > 48) launch_fp<double, 64, 48, 16, 4>(p, st);
