@@ -6,13 +6,18 @@
 // lowest nev eigenpairs on the GPU:
 //   1. Householder reduction to tridiagonal form (one workgroup; the matrix
 //      stays L2-resident, the Householder vector and the product vector in LDS),
-//   2. Sturm-sequence bisection for each wanted eigenvalue (one thread each),
+//   2. Sturm-sequence multisection of the wanted eigenvalues (several points per
+//      eigenvalue and round),
 //   3. inverse iteration with partial-pivot tridiagonal LU, vectors of close
 //      eigenvalues (gap < 1e-3 ||T||) re-orthogonalised within their cluster,
 //   4. back-transformation with the stored reflectors.
 // The non-Hermitian step follows Solver.py:759-798: Mmsq = sqrtm(Mm) (coupled
 // Newton-Schulz iteration on MFMA GEMMs), Mh = Mmsq Mp Mmsq, the lowest nev
-// eigenpairs of Mh, Rb = Mmsq Tb / sqrt(w), Lb = Mp Rb / w.
+// eigenpairs of Mh, Rb = Mmsq Tb / sqrt(w), Lb = Mp Rb / w.  The Davidson
+// iterations take the same roots, Rb and Lb from a blocked Cholesky factor of Mm
+// (dftb_subspace_nonhermitian_rl) and Tb once at the end (dftb_subspace_tb).
+// For l >= EIG_MULTI_L the tridiagonalisation runs over many workgroups, one
+// launch per column; the multisection and inverse iteration stay in one.
 #include <math.h>
 #include "common.h"
 #include "gemm.h"
