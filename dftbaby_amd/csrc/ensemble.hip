@@ -13,7 +13,7 @@
 #include "blockops.h"
 #include "../../include/dftbaby_amd.h"
 
-#define ENS_THREADS 256
+#define ENS_THREADS 512
 #define ENS_MAXL 1024   // largest subspace (LDS scratch of the eigensolver)
 #define ENS_MAXK 32     // largest number of states
 
